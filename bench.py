@@ -1,0 +1,189 @@
+"""Benchmark: FUNWAVE-TVD per-time-step solver on one MI355X.
+
+Workload: the 2-D large-domain beach (2048 x 2048 cells, dx = dy = 2 m,
+1:40 planar beach + Gaussian shoal + band-limited noise, irregular
+directional JONSWAP waves through the WK_DATA2D internal wavemaker, sponges
+on three sides, wetting-drying, dispersion and breaking on), synthetic
+inputs built on the host by fwcases/fwsetup.  A step is one whole time step
+(ESTIMATE_DT + three SSP-RK3 stages); the metric is interior
+cells x steps / second / 1e6.
+
+Prints one JSON line.  ``--gpus`` above 1 is refused: this build runs one
+GPU per domain.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "funwave-tvd_amd"))
+
+METRIC = "Mcell-steps/s at 2048^2; ensemble runs/hr learned vs plain at eta err<=2%"
+HBM_PEAK = 8.0e12           # MI355X HBM3E, B/s
+B_ALG_WORDS = 93            # algorithmic words per interior cell-step (whole step)
+
+# Algorithmic traffic of each kernel per unit of its launch: (real words
+# read, int32 words read, real words written, unit), counting every distinct
+# array element the routine must read or write once.  Units: "cell" =
+# Mloc*Nloc, "xface" = (Mloc+1)*Nloc, "yface" = Mloc*(Nloc+1), "inner" =
+# Mglob*Nglob.
+KERNEL_TRAFFIC = {
+    "flux_x": (8, 2, 3, "xface"),        # U V HU HV Eta U4 V4 DepthX, MASK MASK9 -> P Fx Gx
+    "flux_y": (8, 2, 3, "yface"),
+    "disp_deriv": (9, 1, 29, "cell"),    # U V U0 V0 Eta Depth P Q (+Ut) -> 26 derivatives, ETAT, Ut, Vt
+    "disp_terms": (31, 1, 10, "cell"),   # derivative arrays, Depth, Eta, U, V -> U4 V4 U1p V1p U1pp..V3
+    "sources": (24, 2, 2, "inner"),
+    "huv": (24, 2, 12, "cell"),
+    "trid": (6, 0, 2, "inner"),
+    "uv_final": (3, 1, 4, "cell"),
+    "mask": (4, 2, 2, "cell"),
+    "brk1": (8, 0, 2, "cell"),
+    "mask_mul": (6, 1, 5, "cell"),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", choices=["sp", "dp"], default="sp")
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-size", type=int, default=512)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    return ap.parse_args()
+
+
+def sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def kernel_traffic_bytes(name, rb, M, N, mg, ng):
+    r, i, w, unit = KERNEL_TRAFFIC[name]
+    units = {"cell": M * N, "xface": (M + 1) * N, "yface": M * (N + 1), "inner": mg * ng}[unit]
+    return units * ((r + w) * rb + i * 4)
+
+
+def cpu_baseline(args, precision):
+    """The compiled reference (oracle/_ref) on a bounded sample of the same
+    workload family, timed with its own monotonic clock on one core."""
+    ref = os.path.join(REPO, "oracle", "_ref", f"fwref_{precision}")
+    if not os.path.exists(ref):
+        return None
+    import fwcases
+    n = args.cpu_size
+    case = fwcases.beach_2d(mglob=n, nglob=n, dx=2.0)
+    warm = 2
+    with tempfile.TemporaryDirectory() as td:
+        case.write(td)
+        env = dict(os.environ, FWREF_NSTEPS=str(warm + args.cpu_steps), FWREF_WARMUP=str(warm),
+                   MALLOC_PERTURB_="255", OMP_NUM_THREADS="1")
+        t0 = time.time()
+        r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec taskset -c 0 {ref} input.txt"],
+                           cwd=td, env=env, capture_output=True, text=True, timeout=600)
+        wall = time.time() - t0
+    line = [l for l in r.stdout.splitlines() if l.startswith("FWREF_TIMING")]
+    if r.returncode != 0 or not line:
+        return {"error": f"reference run failed rc={r.returncode}"}
+    import re
+    mt = re.search(r"steps=\s*(\d+)\s+seconds=\s*([0-9.Ee+-]+)", line[0])
+    steps, secs = int(mt.group(1)), float(mt.group(2))
+    return {"value": n * n * steps / secs / 1e6, "unit": "Mcell-steps/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"reference FUNWAVE-TVD ({'FP64' if precision == 'dp' else 'FP32'} serial build, "
+                      f"amdflang -O2) on the same beach workload at {n}x{n} cells, dx=2 m: "
+                      f"{steps} steps after {warm} warm-up steps, {secs:.2f} s timed "
+                      f"({wall:.1f} s with set-up), one core"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported; this build runs one domain on one GPU "
+              "(multi-GPU sharding is design-only).", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np  # noqa: F401
+    import fwcases
+    import fwgpu
+    import fwsetup
+
+    t_setup = time.time()
+    n = args.size
+    case = fwcases.beach_2d(mglob=n, nglob=n, dx=2.0)
+    st = fwsetup.build_state(case, args.precision)
+    g = fwgpu.Solver(st, args.precision)
+    t_setup = time.time() - t_setup
+    rb = 8 if args.precision == "dp" else 4
+    M, N = g.M, g.N
+
+    g.step(args.warmup)
+    sync()
+    g.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = g.time_steps(args.steps)
+    g.synchronize()
+    sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    mcs = n * n * args.steps / wall / 1e6
+
+    # per-kernel device time (HIP events on the solver's stream)
+    g.profile(True)
+    g.profile_reset()
+    g.step(args.profile_steps)
+    kern = {}
+    for name in g.profile_names():
+        cnt, ms = g.profile_get(name)
+        kern[name] = (cnt, ms)
+    g.profile(False)
+    tot = sum(v[1] for v in kern.values())
+    dom = max((k for k in kern if k in KERNEL_TRAFFIC), key=lambda k: kern[k][1])
+    cnt, ms = kern[dom]
+    avg_s = ms / cnt / 1e3
+    alg = kernel_traffic_bytes(dom, rb, M, N, n, n)
+    achieved = alg / avg_s
+    step_s = wall / args.steps
+    step_alg = B_ALG_WORDS * rb * n * n
+    out = {
+        "metric": METRIC, "value": round(mcs, 3), "unit": "Mcell-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if rb == 8 else "f32", "data": "synthetic",
+        "config": {"workload": f"2-D beach + Gaussian shoal + noise, JONSWAP 200-component WK_DATA2D "
+                               f"waves, {n}x{n} cells dx=2 m, sponges W/S/N, wetting-drying, "
+                               f"dispersion + breaking, whole RK3 steps",
+                   "Mglob": n, "Nglob": n, "precision": "FP64" if rb == 8 else "FP32",
+                   "parallelism": "one domain on one GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
+                     "alg_bytes_per_launch": alg},
+        "step_roofline": {"alg_bytes_per_cell_step": B_ALG_WORDS * rb,
+                          "achieved_GBps": round(step_alg / step_s / 1e9, 2),
+                          "frac": round(step_alg / step_s / HBM_PEAK, 4),
+                          "device_ms_per_step": round(dev_ms / args.steps, 4)},
+        "kernels_ms_per_step": {k: round(v[1] / args.profile_steps, 4) for k, v in
+                                sorted(kern.items(), key=lambda kv: -kv[1][1])},
+        "setup_s": round(t_setup, 1),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args, args.precision)
+    g.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1242 @@
+// fw_kernels.h -- HIP kernels of the per-step solver (gfx950).
+//
+// Floating-point contract: every kernel evaluates the reference's
+// expressions with the same operand order and no contraction (the library
+// is built with -ffp-contract=off), with correctly rounded division and
+// square root, so results are bitwise those of the reference's CPU build at
+// the same precision.  The exceptions are the transcendental calls that the
+// reference evaluates per cell (ATAN2 in the breaking angle and the Froude
+// cap, COS/SIN in the Froude cap): those use the device math library and can
+// differ from the host C library in the last bit.
+// Time-only transcendentals (wavemaker ramps and phases) are evaluated on the
+// host with the C library, as the reference does, and passed in.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fw_fields.h"
+
+namespace fw {
+
+#define IX(i, j) ((size_t)((j)-1) * (size_t)g.P + (size_t)((i)-1))
+
+template <class R> struct Lit;
+template <> struct Lit<float> {
+  static __host__ __device__ constexpr float grav() { return 9.81f; }
+  static __host__ __device__ constexpr float small() { return 0.000001f; }
+  static __host__ __device__ constexpr float large() { return 999999.0f; }
+};
+template <> struct Lit<double> {
+  static __host__ __device__ constexpr double grav() { return 9.81; }
+  static __host__ __device__ constexpr double small() { return 0.000001; }
+  static __host__ __device__ constexpr double large() { return 999999.0; }
+};
+// pi is a default-REAL literal in the reference (src/mod_param.F:71)
+#define FW_PI ((R)3.141592653f)
+#define GRAV (Lit<R>::grav())
+#define SMALL (Lit<R>::small())
+
+template <class R> __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
+template <class R> __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }  // correctly rounded (llvm.sqrt.f32)
+__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float dabs(float x) { return fabsf(x); }
+__device__ __forceinline__ double dabs(double x) { return fabs(x); }
+// single-precision transcendentals are evaluated in double and rounded once,
+// i.e. correctly rounded except in rare hard cases
+__device__ __forceinline__ float datan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+__device__ __forceinline__ double datan2(double y, double x) { return atan2(y, x); }
+__device__ __forceinline__ float dcos(float x) { return (float)cos((double)x); }
+__device__ __forceinline__ double dcos(double x) { return cos(x); }
+__device__ __forceinline__ float dsin(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ double dsin(double x) { return sin(x); }
+__device__ __forceinline__ float ddiv(float a, float b) { return __fdiv_rn(a, b); }
+__device__ __forceinline__ double ddiv(double a, double b) { return a / b; }
+// SIGN(1.0, b) with IEEE signed zero
+template <class R> __device__ __forceinline__ R sign1(R b) { return signbit(b) ? (R)-1 : (R)1; }
+
+// --------------------------------------------------------------------------
+// Ghost-cell fills (PHI_COLL bc.F:631-1272, PHI_COLL_VARIABLE_LENGTH
+// bc.F:539-617).  The reference fills x ghosts on interior rows, then y
+// ghosts over all columns, so in the non-periodic case every ghost value is
+// a pure function of one interior value: phi(mi, mj) times the mirror signs.
+// With PERIODIC, the y ghost rows are copied first over all columns (the
+// x-ghost columns still holding their previous values) and the x ghosts are
+// filled afterwards; that takes two ordered launches (modes 1 and 2).
+struct HaloArr {
+  void *p;
+  int vtype;   // 1 even, 2 odd in x, 3 odd in y
+  int isint;
+  int spec;    // 0 cell, 1 x-face (P), 2 y-face (Q)
+};
+struct HaloList {
+  HaloArr a[32];
+  int n;
+};
+
+struct HaloSpec {
+  int Ma, Na, Ib, Ie, Jb, Je;
+};
+
+__device__ __forceinline__ HaloSpec halo_spec(const Geo &g, int spec) {
+  HaloSpec s{g.M, g.N, g.Ib, g.Ie, g.Jb, g.Je};
+  if (spec == 1) { s.Ma = g.M + 1; s.Ie = g.Ie + 1; }
+  if (spec == 2) { s.Na = g.N + 1; s.Je = g.Je + 1; }
+  return s;
+}
+
+// ghost index t -> (i, j); bottom rows, top rows, then left/right columns of
+// interior rows.  Returns false past the end.
+__device__ __forceinline__ bool halo_pos(const HaloSpec &s, int t, int mode, int &i, int &j) {
+  int rows = 3 * s.Ma;
+  int nr = s.Je - s.Jb + 1;
+  if (mode != 2) {
+    if (t < rows) { j = 1 + t / s.Ma; i = 1 + t % s.Ma; return true; }
+    t -= rows;
+    if (t < rows) { j = s.Je + 1 + t / s.Ma; i = 1 + t % s.Ma; return true; }
+    t -= rows;
+    if (mode == 1) return false;
+  }
+  if (t < 3 * nr) { j = s.Jb + t / 3; i = 1 + t % 3; return true; }
+  t -= 3 * nr;
+  if (t < 3 * nr) { j = s.Jb + t / 3; i = s.Ie + 1 + t % 3; return true; }
+  return false;
+}
+
+template <class R>
+__global__ void k_halo(HaloList L, Geo g, int mode) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int q = 0; q < L.n; q++) {
+    const HaloArr A = L.a[q];
+    HaloSpec s = halo_spec(g, A.spec);
+    int i, j;
+    if (!halo_pos(s, t, mode, i, j)) continue;
+    int si = i, sj = j;
+    bool neg = false;
+    if (mode == 1) {  // periodic y copy (raw, no sign)
+      sj = (j <= 3) ? (s.Je - 3 + j) : (3 + (j - s.Je));
+    } else {
+      if (i < s.Ib) { si = 2 * s.Ib - 1 - i; neg ^= (A.vtype == 2); }
+      if (i > s.Ie) { si = 2 * s.Ie + 1 - i; neg ^= (A.vtype == 2); }
+      if (mode == 0) {
+        if (j < s.Jb) { sj = 2 * s.Jb - 1 - j; neg ^= (A.vtype == 3); }
+        if (j > s.Je) { sj = 2 * s.Je + 1 - j; neg ^= (A.vtype == 3); }
+      }
+    }
+    if (A.isint) {
+      int *ph = (int *)A.p;
+      ph[IX(i, j)] = ph[IX(si, sj)];
+    } else {
+      R *ph = (R *)A.p;
+      R v = ph[IX(si, sj)];
+      ph[IX(i, j)] = neg ? -v : v;
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// ESTIMATE_DT (misc.F:216-296) over all cells including ghosts; the block
+// minima are reduced by k_dt_final.  Also U0 := U, V0 := V (main.F:331-332)
+// when copy_uv is set.
+template <class R>
+__global__ void k_dt_partial(Dev<R> d, Geo g, Prm<R> p, int copy_uv) {
+  __shared__ R sm[256];
+  R t3 = Lit<R>::large();
+  int n = g.M * g.N;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    int i = 1 + q % g.M, j = 1 + q / g.M;
+    size_t o = IX(i, j);
+    R u = d.U[o], v = d.V[o], h = d.H[o];
+    if (copy_uv) { d.U0[o] = u; d.V0[o] = v; }
+    R c = dsqrt(GRAV * rmax(h, p.MDF));
+    R t1 = dabs(u) + c;
+    R t2 = (t1 < SMALL) ? ddiv(p.DX, SMALL) : ddiv(p.DX, t1);
+    if (t2 < t3) t3 = t2;
+    t1 = dabs(v) + c;
+    t2 = (t1 < SMALL) ? ddiv(p.DY, SMALL) : ddiv(p.DY, t1);
+    if (t2 < t3) t3 = t2;
+  }
+  sm[threadIdx.x] = t3;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = rmin(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d.red[blockIdx.x] = sm[0];
+}
+
+template <class R>
+__global__ void k_dt_final(Dev<R> d, Prm<R> p, int nblk, R *out) {
+  __shared__ R sm[256];
+  R m = Lit<R>::large();
+  for (int q = threadIdx.x; q < nblk; q += blockDim.x) m = rmin(m, d.red[q]);
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = rmin(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    R dtt = p.CFL * sm[0];
+    R dt;
+    if (p.FIXED_DT) {
+      dt = p.DTF;
+      while (dt > dtt) dt = ddiv(dt, (R)2);
+    } else {
+      dt = dtt;
+    }
+    out[0] = dt;
+  }
+}
+
+// max |eta| over the interior (CHECK_BLOWUP, misc.F:394-433)
+template <class R>
+__global__ void k_maxeta_partial(Dev<R> d, Geo g) {
+  __shared__ R sm[256];
+  R m = (R)0;
+  int nx = g.Ie - g.Ib + 1, n = nx * (g.Je - g.Jb + 1);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    int i = g.Ib + q % nx, j = g.Jb + q / nx;
+    R a = dabs(d.Eta[IX(i, j)]);
+    if (a > m) m = a;
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = rmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d.red[blockIdx.x] = sm[0];
+}
+template <class R>
+__global__ void k_max_final(Dev<R> d, int nblk, R *out) {
+  __shared__ R sm[256];
+  R m = (R)0;
+  for (int q = threadIdx.x; q < nblk; q += blockDim.x) m = rmax(m, d.red[q]);
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = rmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sm[0];
+}
+
+// plain copies of whole pitched arrays (Eta0 := Eta etc.)
+template <class R>
+__global__ void k_copy3(R *d0, const R *s0, R *d1, const R *s1, R *d2, const R *s2, size_t n) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    d0[q] = s0[q];
+    d1[q] = s1[q];
+    d2[q] = s2[q];
+  }
+}
+
+// --------------------------------------------------------------------------
+// CAL_DISPERSION part 1 (dispersion.F:77-287): masked central derivatives
+// (derivatives.F:123-289) on the interior, ETAT on 1..Mloc-1 x 1..Nloc-1,
+// Ut/Vt on all cells, and zeroed cross derivatives on the wall lines.
+// DU = max(Depth,MinDepthFrc) U and DUt are recomputed at each stencil point.
+template <class R>
+__device__ __forceinline__ R dfx(const Geo &g, const R *a, int i, int j, R dx) {
+  return ddiv(ddiv(a[IX(i + 1, j)] - a[IX(i - 1, j)], dx), (R)2);
+}
+template <class R>
+__device__ __forceinline__ R dfy(const Geo &g, const R *a, int i, int j, R dy) {
+  return ddiv(ddiv(a[IX(i, j + 1)] - a[IX(i, j - 1)], dy), (R)2);
+}
+template <class R>
+__device__ __forceinline__ R dfxx(const Geo &g, const R *a, int i, int j, R dx) {
+  return ddiv(ddiv(a[IX(i + 1, j)] - (R)2 * a[IX(i, j)] + a[IX(i - 1, j)], dx), dx);
+}
+template <class R>
+__device__ __forceinline__ R dfyy(const Geo &g, const R *a, int i, int j, R dy) {
+  return ddiv(ddiv(a[IX(i, j + 1)] - (R)2 * a[IX(i, j)] + a[IX(i, j - 1)], dy), dy);
+}
+template <class R>
+__device__ __forceinline__ R dfxy(const Geo &g, const R *a, int i, int j, R dx, R dy) {
+  R t1 = ddiv(ddiv(a[IX(i + 1, j + 1)] - a[IX(i + 1, j - 1)], (R)2), dy);
+  R t2 = ddiv(ddiv(a[IX(i - 1, j + 1)] - a[IX(i - 1, j - 1)], (R)2), dy);
+  return ddiv(ddiv(t1 - t2, (R)2), dx);
+}
+// stencil of a derived field f(i,j) evaluated on the fly
+template <class R, class Fn>
+__device__ __forceinline__ R gfx(Fn f, int i, int j, R dx) { return ddiv(ddiv(f(i + 1, j) - f(i - 1, j), dx), (R)2); }
+template <class R, class Fn>
+__device__ __forceinline__ R gfy(Fn f, int i, int j, R dy) { return ddiv(ddiv(f(i, j + 1) - f(i, j - 1), dy), (R)2); }
+template <class R, class Fn>
+__device__ __forceinline__ R gfxx(Fn f, int i, int j, R dx) { return ddiv(ddiv(f(i + 1, j) - (R)2 * f(i, j) + f(i - 1, j), dx), dx); }
+template <class R, class Fn>
+__device__ __forceinline__ R gfyy(Fn f, int i, int j, R dy) { return ddiv(ddiv(f(i, j + 1) - (R)2 * f(i, j) + f(i, j - 1), dy), dy); }
+template <class R, class Fn>
+__device__ __forceinline__ R gfxy(Fn f, int i, int j, R dx, R dy) {
+  R t1 = ddiv(ddiv(f(i + 1, j + 1) - f(i + 1, j - 1), (R)2), dy);
+  R t2 = ddiv(ddiv(f(i - 1, j + 1) - f(i - 1, j - 1), (R)2), dy);
+  return ddiv(ddiv(t1 - t2, (R)2), dx);
+}
+
+template <class R>
+__global__ void k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.M || j > g.N) return;
+  size_t o = IX(i, j);
+  const R dx = p.DX, dy = p.DY, mdf = p.MDF;
+  const bool g2 = p.G2 > (R)0;
+  // ETAT from the previous stage's P, Q (dispersion.F:103-119)
+  if (i <= g.M - 1 && j <= g.N - 1) {
+    d.ETAT[o] = -ddiv(d.P[IX(i + 1, j)] - d.P[IX(i, j)], dx) - ddiv(d.Q[IX(i, j + 1)] - d.Q[IX(i, j)], dy);
+  }
+  if (g2) {  // :133-142
+    d.Ut[o] = ddiv(d.U[o] - d.U0[o], DT);
+    d.Vt[o] = ddiv(d.V[o] - d.V0[o], DT);
+  }
+  if (i < g.Ib || i > g.Ie || j < g.Jb || j > g.Je) return;
+  const R m9 = (R)d.MASK9[o];
+  const R *U = d.U, *V = d.V, *Dp = d.Depth, *U0 = d.U0, *V0 = d.V0;
+  auto DU = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * U[q]; };
+  auto DV = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * V[q]; };
+  auto UT = [&](int a, int b) { size_t q = IX(a, b); return ddiv(U[q] - U0[q], DT); };
+  auto VT = [&](int a, int b) { size_t q = IX(a, b); return ddiv(V[q] - V0[q], DT); };
+  auto DUT = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * ddiv(U[q] - U0[q], DT); };
+  auto DVT = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * ddiv(V[q] - V0[q], DT); };
+  const bool wall = (i == g.Ib || i == g.Ie || j == g.Jb || j == g.Je);
+  d.Uxx[o] = dfxx(g, U, i, j, dx) * m9;
+  d.Uxy[o] = wall ? (R)0 : dfxy(g, U, i, j, dx, dy) * m9;
+  d.Vxy[o] = wall ? (R)0 : dfxy(g, V, i, j, dx, dy) * m9;
+  d.Vyy[o] = dfyy(g, V, i, j, dy) * m9;
+  d.DUxx[o] = gfxx<R>(DU, i, j, dx) * m9;
+  d.DUxy[o] = wall ? (R)0 : gfxy<R>(DU, i, j, dx, dy) * m9;
+  d.DVxy[o] = wall ? (R)0 : gfxy<R>(DV, i, j, dx, dy) * m9;
+  d.DVyy[o] = gfyy<R>(DV, i, j, dy) * m9;
+  if (g2) {
+    d.Ux[o] = dfx(g, U, i, j, dx) * m9;
+    d.Vx[o] = dfx(g, V, i, j, dx) * m9;
+    d.Uy[o] = dfy(g, U, i, j, dy) * m9;
+    d.Vy[o] = dfy(g, V, i, j, dy) * m9;
+    d.ETAx[o] = dfx(g, d.Eta, i, j, dx) * m9;
+    d.ETAy[o] = dfy(g, d.Eta, i, j, dy) * m9;
+    d.DUx[o] = gfx<R>(DU, i, j, dx) * m9;
+    d.DVx[o] = gfx<R>(DV, i, j, dx) * m9;
+    d.DUy[o] = gfy<R>(DU, i, j, dy) * m9;
+    d.DVy[o] = gfy<R>(DV, i, j, dy) * m9;
+    d.Utx[o] = gfx<R>(UT, i, j, dx) * m9;
+    d.Vty[o] = gfy<R>(VT, i, j, dy) * m9;
+    d.Utxx[o] = gfxx<R>(UT, i, j, dx) * m9;
+    d.Vtyy[o] = gfyy<R>(VT, i, j, dy) * m9;
+    d.Utxy[o] = wall ? (R)0 : gfxy<R>(UT, i, j, dx, dy) * m9;
+    d.Vtxy[o] = wall ? (R)0 : gfxy<R>(VT, i, j, dx, dy) * m9;
+    d.DUtx[o] = gfx<R>(DUT, i, j, dx) * m9;
+    d.DVty[o] = gfy<R>(DVT, i, j, dy) * m9;
+    d.DUtxx[o] = gfxx<R>(DUT, i, j, dx) * m9;
+    d.DVtyy[o] = gfyy<R>(DVT, i, j, dy) * m9;
+    d.DUtxy[o] = wall ? (R)0 : gfxy<R>(DUT, i, j, dx, dy) * m9;
+    d.DVtxy[o] = wall ? (R)0 : gfxy<R>(DVT, i, j, dx, dy) * m9;
+  } else if (p.SHOW_BRK) {
+    d.ETAx[o] = dfx(g, d.Eta, i, j, dx) * m9;
+    d.ETAy[o] = dfy(g, d.Eta, i, j, dy) * m9;
+  }
+}
+
+// CAL_DISPERSION part 2: U4, V4, U1p, V1p on all cells (:293-330) and the
+// nonlinear terms U1pp..V3 on the interior (:398-503).
+template <class R>
+__global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.M || j > g.N) return;
+  size_t o = IX(i, j);
+  const R B1 = p.B1, B2 = p.B2, b2 = p.b2, g2 = p.G2, dx = p.DX, dy = p.DY;
+  const R c13 = (R)1 / (R)3, c12 = (R)1 / (R)2, c16 = (R)1 / (R)6;
+  {
+    R dep = d.Depth[o];
+    R uxx = d.Uxx[o], vxy = d.Vxy[o], uxy = d.Uxy[o], vyy = d.Vyy[o];
+    R duxx = d.DUxx[o], dvxy = d.DVxy[o], duxy = d.DUxy[o], dvyy = d.DVyy[o];
+    R k0 = c13 - B1 + (R)0.5 * B1 * B1;
+    R u4 = k0 * dep * dep * (uxx + vxy) + (B1 - c12) * dep * (duxx + dvxy);
+    R v4 = k0 * dep * dep * (uxy + vyy) + (B1 - c12) * dep * (duxy + dvyy);
+    if (g2 > (R)0) {
+      R reta = d.Eta[o];
+      R ken1 = (c16 - B1 + B1 * B1) * dep * reta * B2 + (c12 * B1 * B1 - c16) * reta * reta * B2 * B2;
+      R ken2 = (B1 - c12) * reta * B2;
+      R m9 = (R)d.MASK9[o];
+      u4 = u4 + g2 * m9 * (ken1 * (uxx + vxy) + ken2 * (duxx + dvxy));
+      v4 = v4 + g2 * m9 * (ken1 * (uxy + vyy) + ken2 * (duxy + dvyy));
+    }
+    d.U4[o] = u4;
+    d.V4[o] = v4;
+    d.U1p[o] = (R)0.5 * ((R)1 - B1) * ((R)1 - B1) * dep * dep * (uxx + vxy) + (B1 - (R)1) * dep * (duxx + dvxy);
+    d.V1p[o] = (R)0.5 * ((R)1 - B1) * ((R)1 - B1) * dep * dep * (uxy + vyy) + (B1 - (R)1) * dep * (duxy + dvyy);
+  }
+  if (!(g2 > (R)0)) return;
+  if (i < g.Ib || i > g.Ie || j < g.Jb || j > g.Je) return;
+#define A_(n, a, b) d.n[IX(a, b)]
+  R UxxVxy = A_(Uxx, i, j) + A_(Vxy, i, j);
+  R UxyVyy = A_(Uxy, i, j) + A_(Vyy, i, j);
+  R UxxVxy_x = ddiv(ddiv(A_(Uxx, i + 1, j) + A_(Vxy, i + 1, j) - A_(Uxx, i - 1, j) - A_(Vxy, i - 1, j), (R)2), dx);
+  R UxxVxy_y = ddiv(ddiv(A_(Uxx, i, j + 1) + A_(Vxy, i, j + 1) - A_(Uxx, i, j - 1) - A_(Vxy, i, j - 1), (R)2), dy);
+  R UxyVyy_x = ddiv(ddiv(A_(Uxy, i + 1, j) + A_(Vyy, i + 1, j) - A_(Uxy, i - 1, j) - A_(Vyy, i - 1, j), (R)2), dx);
+  R UxyVyy_y = ddiv(ddiv(A_(Uxy, i, j + 1) + A_(Vyy, i, j + 1) - A_(Uxy, i, j - 1) - A_(Vyy, i, j - 1), (R)2), dy);
+  R HUxxHVxy = A_(DUxx, i, j) + A_(DVxy, i, j);
+  R HUxyHVyy = A_(DUxy, i, j) + A_(DVyy, i, j);
+  R HUxxHVxy_x = ddiv(ddiv(A_(DUxx, i + 1, j) + A_(DVxy, i + 1, j) - A_(DUxx, i - 1, j) - A_(DVxy, i - 1, j), (R)2), dx);
+  R HUxxHVxy_y = ddiv(ddiv(A_(DUxx, i, j + 1) + A_(DVxy, i, j + 1) - A_(DUxx, i, j - 1) - A_(DVxy, i, j - 1), (R)2), dy);
+  R HUxyHVyy_x = ddiv(ddiv(A_(DUxy, i + 1, j) + A_(DVyy, i + 1, j) - A_(DUxy, i - 1, j) - A_(DVyy, i - 1, j), (R)2), dx);
+  R HUxyHVyy_y = ddiv(ddiv(A_(DUxy, i, j + 1) + A_(DVyy, i, j + 1) - A_(DUxy, i, j - 1) - A_(DVyy, i, j - 1), (R)2), dy);
+  R rh = A_(Depth, i, j);
+  R rhx = ddiv(ddiv(A_(Depth, i + 1, j) - A_(Depth, i - 1, j), (R)2), dx);
+  R rhy = ddiv(ddiv(A_(Depth, i, j + 1) - A_(Depth, i, j - 1), (R)2), dy);
+  R reta = A_(Eta, i, j);
+  R ex = A_(ETAx, i, j), ey = A_(ETAy, i, j), et = A_(ETAT, i, j);
+  R Utx_ = A_(Utx, i, j), Vty_ = A_(Vty, i, j);
+  R u1pp = -reta * B2 * ex * B2 * (Utx_ + Vty_) - (R)0.5 * reta * reta * B2 * B2 * (A_(Utxx, i, j) + A_(Vtxy, i, j))
+           - ex * B2 * (A_(DUtx, i, j) + A_(DVty, i, j)) - reta * B2 * (A_(DUtxx, i, j) + A_(DVtxy, i, j));
+  R v1pp = -reta * B2 * ey * B2 * (Utx_ + Vty_) - (R)0.5 * reta * reta * B2 * B2 * (A_(Utxy, i, j) + A_(Vtyy, i, j))
+           - ey * B2 * (A_(DUtx, i, j) + A_(DVty, i, j)) - reta * B2 * (A_(DUtxy, i, j) + A_(DVtyy, i, j));
+  R ken1 = B1 * ((R)1 - B1) * rh * et * B2 - B1 * B1 * reta * B2 * et * B2;
+  R ken2 = B1 * ((R)1 - B1) * rh * reta * B2 - (R)0.5f * B1 * B1 * reta * reta * B2 * B2;
+  R ken3 = B1 * et * B2;
+  R ken4 = B1 * reta * B2;
+  u1pp = u1pp - ken1 * UxxVxy - ken2 * (A_(Utxx, i, j) + A_(Vtxy, i, j)) + ken3 * HUxxHVxy + ken4 * (A_(DUtxx, i, j) + A_(DVtxy, i, j));
+  v1pp = v1pp - ken1 * UxyVyy - ken2 * (A_(Utxy, i, j) + A_(Vtyy, i, j)) + ken3 * HUxyHVyy + ken4 * (A_(DUtxy, i, j) + A_(DVtyy, i, j));
+  d.U1pp[o] = u1pp;
+  d.V1pp[o] = v1pp;
+  R omB = (R)1 - B1;
+  ken1 = (B1 - (R)1) * (rhx + ex) * B2;
+  ken2 = (B1 - (R)1) * (rh + reta) * B2;
+  ken3 = (omB * omB * rh * rhx * B2 * B2 - B1 * omB * (rhx * reta * B2 + rh * ex * B2) + (B1 * B1 - (R)1) * reta * ex * B2 * B2);
+  ken4 = ((R)0.5 * omB * omB * rh * rh * B2 * B2 - B1 * omB * rh * reta * B2 + (R)0.5 * (B1 * B1 - (R)1) * reta * reta * B2 * B2);
+  R ken5 = (omB * omB * rh * rhy * B2 * B2 - B1 * omB * (rhy * reta * B2 + rh * ey * B2) + (B1 * B1 - (R)1) * reta * ey * B2 * B2);
+  R u = A_(U, i, j), v = A_(V, i, j);
+  R ux = A_(Ux, i, j), vx = A_(Vx, i, j), uy = A_(Uy, i, j), vy = A_(Vy, i, j);
+  R dux = A_(DUx, i, j), dvy = A_(DVy, i, j);
+  d.U2[o] = ken1 * (u * HUxxHVxy + v * HUxyHVyy) + ken2 * (ux * HUxxHVxy + u * HUxxHVxy_x + vx * HUxyHVyy + v * HUxyHVyy_x)
+            + ken3 * (u * UxxVxy + v * UxyVyy) + ken4 * (ux * UxxVxy + u * UxxVxy_x + vx * UxyVyy + v * UxyVyy_x)
+            + B2 * B2 * (dux + dvy + reta * B2 * (ux + vy)) * (HUxxHVxy + ex * B2 * (ux + vy) + reta * B2 * UxxVxy);
+  ken1 = (B1 - (R)1) * (rhy + ey) * B2;
+  d.V2[o] = ken1 * (u * HUxxHVxy + v * HUxyHVyy) + ken2 * (uy * HUxxHVxy + u * HUxxHVxy_y + vy * HUxyHVyy + v * HUxyHVyy_y)
+            + ken5 * (u * UxxVxy + v * UxyVyy) + ken4 * (uy * UxxVxy + u * UxxVxy_y + vy * UxyVyy + v * UxyVyy_y)
+            + B2 * B2 * (dux + dvy + reta * B2 * (ux + vy)) * (HUxyHVyy + ey * B2 * (ux + vy) + reta * B2 * UxyVyy);
+  R omega_0 = vx - uy;
+  R omega_1 = (b2 * rhx + B1 * ex) * B2 * (HUxyHVyy + (b2 * rh + B1 * reta) * B2 * UxyVyy)
+              - (b2 * rhy + B1 * ey) * B2 * (HUxxHVxy + (b2 * rh + B1 * reta) * B2 * UxxVxy);
+  ken1 = ((B1 - c12) * (reta + rh) * B2);
+  ken2 = (c13 - B1 + (R)0.5 * B1 * B1) * rh * rh * B2 * B2 + (c16 - B1 + B1 * B1) * rh * reta * B2
+         + (c12 * B1 * B1 - c16) * reta * reta * B2 * B2;
+  d.U3[o] = -v * omega_1 - omega_0 * (ken1 * HUxyHVyy + ken2 * UxyVyy);
+  d.V3[o] = u * omega_1 + omega_0 * (ken1 * HUxxHVxy + ken2 * UxxVxy);
+#undef A_
+}
+
+// --------------------------------------------------------------------------
+// FLUXES (fluxes.F:52-117).  CONSTRUCT_HO_X/Y (:1531-1754): 4th-order
+// MUSCL-TVD with compressive minmod on the 4th-order part and van Leer on
+// the 3rd-order part.
+template <class R>
+__device__ __forceinline__ R mm3(R a, R b, R c) {
+  R tb = (R)2 * b, tc = (R)2 * c;
+  if (a >= (R)0) return rmax((R)0, rmin(rmin(a, tb), tc));
+  return rmin((R)0, rmax(rmax(a, tb), tc));
+}
+template <class R>
+__device__ __forceinline__ R din_val(R vm2, R vm1, R v0, R vp1, int mm2, int mm1, int m0, int mp1) {
+  R t1 = vm1 - vm2, t2 = v0 - vm1, t3 = vp1 - v0;
+  R d1 = mm3(t1, t2, t3), d2 = mm3(t2, t3, t1), d3 = mm3(t3, t1, t2);
+  if (mm2 == 0 || mp1 == 0) {
+    t1 = t2;
+    t3 = t2;
+    d1 = mm3(t1, t2, t3);
+    d2 = mm3(t2, t3, t1);
+    d3 = mm3(t3, t1, t2);
+  }
+  if (mm1 == 0 || m0 == 0) { d1 = (R)0; d2 = (R)0; d3 = (R)0; }
+  return t2 - (R)1 / (R)6 * (d3 - (R)2 * d2 + d1);
+}
+template <class R>
+__device__ __forceinline__ void vanleer(R a, R b, R &v1, R &v2, R &ta, R &tb) {
+  if (dabs(a) <= SMALL) a = SMALL * sign1(a);
+  if (dabs(b) <= SMALL) b = SMALL * sign1(b);
+  R rat = ddiv(b, a);
+  v1 = (R)0;
+  if (dabs((R)1 + rat) > SMALL) v1 = ddiv(rat + dabs(rat), (R)1 + rat);
+  rat = ddiv(a, b);
+  v2 = (R)0;
+  if (dabs((R)1 + rat) > SMALL) v2 = ddiv(rat + dabs(rat), (R)1 + rat);
+  ta = a;
+  tb = b;
+}
+// left/right states at x-face (i,j) from cells i-3..i+2 (stride s along the
+// line: 1 for x, pitch for y)
+template <class R>
+__device__ __forceinline__ void muscl(const R *v, const int *mk, size_t o, ptrdiff_t s, R &outL, R &outR) {
+  // cells c-3..c+2 relative to face between c-1 and c (o = index of cell c)
+  R vv[6];
+  int mm[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    vv[k] = v[o + (k - 3) * s];
+    mm[k] = mk[o + (k - 3) * s];
+  }
+  // Din at cells c-1 (uses c-3..c), c (c-2..c+1), c+1 (c-1..c+2)
+  R dm1 = din_val(vv[0], vv[1], vv[2], vv[3], mm[0], mm[1], mm[2], mm[3]);
+  R d0 = din_val(vv[1], vv[2], vv[3], vv[4], mm[1], mm[2], mm[3], mm[4]);
+  R dp1 = din_val(vv[2], vv[3], vv[4], vv[5], mm[2], mm[3], mm[4], mm[5]);
+  R w1, w2, a, b;
+  vanleer(dm1, d0, w1, w2, a, b);
+  outL = vv[2] + ((R)1 / (R)6) * (w1 * a + (R)2 * w2 * b);
+  vanleer(d0, dp1, w1, w2, a, b);
+  outR = vv[3] - ((R)1 / (R)6) * ((R)2 * w1 * a + w2 * b);
+}
+
+template <class R>
+__device__ __forceinline__ R hll(R sl, R sr, R fl, R fr, R ul, R ur) {
+  if (sl >= (R)0) return fl;
+  if (sr <= (R)0) return fr;
+  R o = sr * fl - sl * fr + sl * sr * (ur - ul);
+  if (dabs(sr - sl) < SMALL) return ddiv(o, SMALL);
+  return ddiv(o, sr - sl);
+}
+template <class R>
+__device__ __forceinline__ R hydro(R xi, R de, R g3) {
+  return (R)0.5 * GRAV * (xi * xi * g3 + (R)2 * xi * de);
+}
+
+// x faces: states, wave speeds (WAVE_SPEED :352-439), HLL (:150-194) and
+// BOUNDARY_CONDITION (bc.F:53-295) resolved per face.
+template <class R>
+__device__ __forceinline__ void xface_speed(const Dev<R> &d, const Geo &g, const Prm<R> &p, int i, int j, R &sl, R &sr) {
+  size_t o = IX(i, j);
+  R ul, ur, el, er;
+  muscl(d.U, d.MASK, o, 1, ul, ur);
+  muscl(d.Eta, d.MASK, o, 1, el, er);
+  R hl = el + d.DepthX[o], hr = er + d.DepthX[o];
+  R spl = dsqrt(GRAV * dabs(hl)), spr = dsqrt(GRAV * dabs(hr));
+  R sps = (R)0.5 * (spl + spr) + (R)0.25 * (ul - ur);
+  R us = (R)0.5 * (ul + ur) + spl - spr;
+  sl = rmin(ul - spl, us - sps);
+  sr = rmax(ur + spr, us + sps);
+}
+template <class R>
+__device__ __forceinline__ void yface_speed(const Dev<R> &d, const Geo &g, const Prm<R> &p, int i, int j, R &sl, R &sr) {
+  size_t o = IX(i, j);
+  R vl, vr, el, er;
+  muscl(d.V, d.MASK, o, (ptrdiff_t)g.P, vl, vr);
+  muscl(d.Eta, d.MASK, o, (ptrdiff_t)g.P, el, er);
+  R hl = el + d.DepthY[o], hr = er + d.DepthY[o];
+  R spl = dsqrt(GRAV * dabs(hl)), spr = dsqrt(GRAV * dabs(hr));
+  R sps = (R)0.5 * (spl + spr) + (R)0.25 * (vl - vr);
+  R us = (R)0.5 * (vl + vr) + spl - spr;
+  sl = rmin(vl - spl, us - sps);
+  sr = rmax(vr + spr, us + sps);
+}
+
+template <class R>
+__global__ void k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int M1 = g.M + 1;
+  if (i > M1 || j > g.N) return;
+  size_t o = IX(i, j);
+  const R g1 = p.G1, g3 = p.G3;
+  const bool comp = (i >= g.Ib && i <= g.Ie + 1 && j >= g.Jb && j <= g.Je);
+  R uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R;
+  if (comp) {
+    muscl(d.U, d.MASK, o, 1, uL, uR);
+    muscl(d.V, d.MASK, o, 1, vL, vR);
+    muscl(d.HU, d.MASK, o, 1, huL, huR);
+    muscl(d.HV, d.MASK, o, 1, hvL, hvR);
+    muscl(d.Eta, d.MASK, o, 1, eL, eR);
+    d.UxL[o] = uL; d.UxR[o] = uR; d.VxL[o] = vL; d.VxR[o] = vR;
+    d.HUxL[o] = huL; d.HUxR[o] = huR; d.HVxL[o] = hvL; d.HVxR[o] = hvR;
+    d.EtaRxL[o] = eL; d.EtaRxR[o] = eR;
+    if (p.DISPERSION) {
+      muscl(d.U4, d.MASK, o, 1, u4L, u4R);
+      muscl(d.V4, d.MASK, o, 1, v4L, v4R);
+      d.U4xL[o] = u4L; d.U4xR[o] = u4R; d.V4xL[o] = v4L; d.V4xR[o] = v4R;
+    } else {
+      u4L = d.U4xL[o]; u4R = d.U4xR[o]; v4L = d.V4xL[o]; v4R = d.V4xR[o];
+    }
+  } else {
+    uL = d.UxL[o]; uR = d.UxR[o]; vL = d.VxL[o]; vR = d.VxR[o];
+    huL = d.HUxL[o]; huR = d.HUxR[o]; hvL = d.HVxL[o]; hvR = d.HVxR[o];
+    eL = d.EtaRxL[o]; eR = d.EtaRxR[o];
+    u4L = d.U4xL[o]; u4R = d.U4xR[o]; v4L = d.V4xL[o]; v4R = d.V4xR[o];
+  }
+  const R dpx = d.DepthX[o];
+  R hL = eL + dpx, hR = eR + dpx;
+  d.HxL[o] = hL; d.HxR[o] = hR;
+  const R m9u = (R)d.MASK9[IX(i <= g.M ? i : g.M, j)];
+  R pl = huL + g1 * m9u * hL * u4L;
+  R pr = huR + g1 * m9u * hR * u4R;
+  R fxl = g3 * pl * (uL + g1 * m9u * u4L) + (R)0.5 * GRAV * (eL * eL * g3 + (R)2 * eL * dpx);
+  R fxr = g3 * pr * (uR + g1 * m9u * u4R) + (R)0.5 * GRAV * (eR * eR * g3 + (R)2 * eR * dpx);
+  R gxl = hL * (uL + g1 * m9u * u4L) * (vL + g1 * m9u * v4L) * g3;
+  R gxr = hR * (uR + g1 * m9u * u4R) * (vR + g1 * m9u * v4R) * g3;
+  d.PL[o] = pl; d.PR[o] = pr; d.FxL[o] = fxl; d.FxR[o] = fxr; d.GxL[o] = gxl; d.GxR[o] = gxr;
+  // wave speed of the nearest computed face (ghost faces copy it)
+  int ci = i < 4 ? 4 : (i > M1 - 3 ? M1 - 3 : i);
+  int cj = j < 4 ? 4 : (j > g.N - 3 ? g.N - 3 : j);
+  R sl, sr;
+  if (ci == i && cj == j) {
+    R spl = dsqrt(GRAV * dabs(hL)), spr = dsqrt(GRAV * dabs(hR));
+    R sps = (R)0.5 * (spl + spr) + (R)0.25 * (uL - uR);
+    R us = (R)0.5 * (uL + uR) + spl - spr;
+    sl = rmin(uL - spl, us - sps);
+    sr = rmax(uR + spr, us + sps);
+  } else {
+    xface_speed(d, g, p, ci, cj, sl, sr);
+  }
+  d.SxL[o] = sl; d.SxR[o] = sr;
+  R P = hll(sl, sr, pl, pr, eL, eR);
+  R Fx = hll(sl, sr, fxl, fxr, huL, huR);
+  R Gx = hll(sl, sr, gxl, gxr, hvL, hvR);
+  // walls (bc.F:86-140)
+  if (j >= g.Jb && j <= g.Je) {
+    if (i == g.Ib) { P = (R)0; Fx = hydro(eR, dpx, g3); Gx = (R)0; }
+    if (i == g.Ie + 1) { P = (R)0; Fx = hydro(eL, dpx, g3); Gx = (R)0; }
+  }
+  // faces of dry cells (bc.F:221-293); the later cell in loop order wins
+  if (j >= g.Jb - 1 && j <= g.Je + 1) {
+    int cl = i - 1, cr = i;
+    if (cl >= g.Ib - 1 && cl <= g.Ie + 1 && d.MASK[IX(cl, j)] < 1) {
+      P = (R)0;
+      Fx = (cl != g.Ie) ? hydro(eR, dpx, g3) * (R)d.MASK[IX(cr, j)] : (R)0;
+      Gx = (R)0;
+    }
+    if (cr >= g.Ib - 1 && cr <= g.Ie + 1 && cr <= g.M && d.MASK[IX(cr, j)] < 1) {
+      P = (R)0;
+      Fx = (cr != g.Ib) ? hydro(eL, dpx, g3) * (R)d.MASK[IX(cl, j)] : (R)0;
+      Gx = (R)0;
+    }
+  }
+  d.P[o] = P; d.Fx[o] = Fx; d.Gx[o] = Gx;
+}
+
+template <class R>
+__global__ void k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int N1 = g.N + 1;
+  if (i > g.M || j > N1) return;
+  size_t o = IX(i, j);
+  const ptrdiff_t s = (ptrdiff_t)g.P;
+  const R g1 = p.G1, g3 = p.G3;
+  const bool comp = (i >= g.Ib && i <= g.Ie && j >= g.Jb && j <= g.Je + 1);
+  R uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R;
+  if (comp) {
+    muscl(d.U, d.MASK, o, s, uL, uR);
+    muscl(d.V, d.MASK, o, s, vL, vR);
+    muscl(d.HV, d.MASK, o, s, hvL, hvR);
+    muscl(d.HU, d.MASK, o, s, huL, huR);
+    muscl(d.Eta, d.MASK, o, s, eL, eR);
+    d.UyL[o] = uL; d.UyR[o] = uR; d.VyL[o] = vL; d.VyR[o] = vR;
+    d.HUyL[o] = huL; d.HUyR[o] = huR; d.HVyL[o] = hvL; d.HVyR[o] = hvR;
+    d.EtaRyL[o] = eL; d.EtaRyR[o] = eR;
+    if (p.DISPERSION) {
+      muscl(d.V4, d.MASK, o, s, v4L, v4R);
+      muscl(d.U4, d.MASK, o, s, u4L, u4R);
+      d.U4yL[o] = u4L; d.U4yR[o] = u4R; d.V4yL[o] = v4L; d.V4yR[o] = v4R;
+    } else {
+      u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
+    }
+  } else {
+    uL = d.UyL[o]; uR = d.UyR[o]; vL = d.VyL[o]; vR = d.VyR[o];
+    huL = d.HUyL[o]; huR = d.HUyR[o]; hvL = d.HVyL[o]; hvR = d.HVyR[o];
+    eL = d.EtaRyL[o]; eR = d.EtaRyR[o];
+    u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
+  }
+  const R dpy = d.DepthY[o];
+  R hL = eL + dpy, hR = eR + dpy;
+  d.HyL[o] = hL; d.HyR[o] = hR;
+  const R m9v = (R)d.MASK9[IX(i, j <= g.N ? j : g.N)];
+  R ql = hvL + g1 * m9v * hL * v4L;
+  R qr = hvR + g1 * m9v * hR * v4R;
+  R gyl = g3 * ql * (vL + g1 * m9v * v4L) + (R)0.5 * GRAV * (eL * eL * g3 + (R)2 * eL * dpy);
+  R gyr = g3 * qr * (vR + g1 * m9v * v4R) + (R)0.5 * GRAV * (eR * eR * g3 + (R)2 * eR * dpy);
+  R fyl = hL * (uL + g1 * m9v * u4L) * (vL + g1 * m9v * v4L) * g3;
+  R fyr = hR * (uR + g1 * m9v * u4R) * (vR + g1 * m9v * v4R) * g3;
+  d.QL[o] = ql; d.QR[o] = qr; d.GyL[o] = gyl; d.GyR[o] = gyr; d.FyL[o] = fyl; d.FyR[o] = fyr;
+  int ci = i < 4 ? 4 : (i > g.M - 3 ? g.M - 3 : i);
+  int cj = j < 4 ? 4 : (j > N1 - 3 ? N1 - 3 : j);
+  R sl, sr;
+  if (ci == i && cj == j) {
+    R spl = dsqrt(GRAV * dabs(hL)), spr = dsqrt(GRAV * dabs(hR));
+    R sps = (R)0.5 * (spl + spr) + (R)0.25 * (vL - vR);
+    R us = (R)0.5 * (vL + vR) + spl - spr;
+    sl = rmin(vL - spl, us - sps);
+    sr = rmax(vR + spr, us + sps);
+  } else {
+    yface_speed(d, g, p, ci, cj, sl, sr);
+  }
+  d.SyL[o] = sl; d.SyR[o] = sr;
+  R Q = hll(sl, sr, ql, qr, eL, eR);
+  R Fy = hll(sl, sr, fyl, fyr, huL, huR);
+  R Gy = hll(sl, sr, gyl, gyr, hvL, hvR);
+  if (!p.PERIODIC && i >= g.Ib && i <= g.Ie) {  // bc.F:147-210
+    if (j == g.Jb) { Q = (R)0; Fy = (R)0; Gy = hydro(eR, dpy, g3); }
+    if (j == g.Je + 1) { Q = (R)0; Fy = (R)0; Gy = hydro(eL, dpy, g3); }
+  }
+  if (i >= g.Ib - 1 && i <= g.Ie + 1) {
+    int cb = j - 1, ct = j;
+    if (cb >= g.Jb - 1 && cb <= g.Je + 1 && d.MASK[IX(i, cb)] < 1) {
+      Q = (R)0;
+      Fy = (R)0;
+      Gy = (cb != g.Je) ? hydro(eR, dpy, g3) * (R)d.MASK[IX(i, ct)] : (R)0;
+    }
+    if (ct >= g.Jb - 1 && ct <= g.Je + 1 && ct <= g.N && d.MASK[IX(i, ct)] < 1) {
+      Q = (R)0;
+      Fy = (R)0;
+      Gy = (ct != g.Jb) ? hydro(eL, dpy, g3) * (R)d.MASK[IX(i, cb)] : (R)0;
+    }
+  }
+  d.Q[o] = Q; d.Fy[o] = Fy; d.Gy[o] = Gy;
+}
+
+// --------------------------------------------------------------------------
+// SourceTerms (sources.F:56-548, Cartesian): wavemaker mass source in the
+// band, then SourceX/SourceY with slope, friction, dispersion, wavemaker and
+// breaking-diffusion terms.  nu_vis is a default REAL (single) array in the
+// reference (:69), so the viscosity sums are formed in single precision.
+template <class R>
+__global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
+  int i = g.Ib + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = g.Jb + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.Ie || j > g.Je) return;
+  size_t o = IX(i, j);
+  const R dx = p.DX, dy = p.DY, mdf = p.MDF, g1 = p.G1, g2 = p.G2;
+  R wm;
+  if (p.wavemaker != 0 && i >= p.ilo && i <= p.ihi && j >= p.jlo && j <= p.jhi) {
+    int bi = i - p.ilo, bj = j - p.jlo;
+    if (p.wavemaker == 1) {
+      wm = AA * d.wk_exp[bi] * d.wk_stage[bj];
+    } else {
+      int bw = p.ihi - p.ilo + 1, bh = p.jhi - p.jlo + 1;
+      size_t plane = (size_t)bw * bh, q = (size_t)bj * bw + bi;
+      const R *bb = d.wk_stage, *cc = d.wk_stage + p.NumFreq;
+      R s = (R)0;
+      for (int k = 0; k < p.NumFreq; k++) s = s + AA * (d.Cm[q + k * plane] * bb[k] + d.Sm[q + k * plane] * cc[k]);
+      wm = s;
+    }
+    d.WaveMaker_Mass[o] = wm;
+  } else {
+    wm = d.WaveMaker_Mass[o];
+  }
+  R u = d.U[o], v = d.V[o];
+  R spd = dsqrt(u * u + v * v);
+  R fix = -(d.Cd[o] * u * spd);
+  d.FrcInsX[o] = fix;
+  R m9r = (R)d.MASK9[o];
+  R hmax = rmax(d.H[o], mdf);
+  R divpq = ddiv(d.P[IX(i + 1, j)] - d.P[o], dx) + ddiv(d.Q[IX(i, j + 1)] - d.Q[o], dy);
+  const R *U4 = d.U4, *V4 = d.V4, *U = d.U, *V = d.V;
+  R mk = (R)d.MASK[o];
+  R sx = ddiv(GRAV * (d.Eta[o]), dx) * (d.DepthX[IX(i + 1, j)] - d.DepthX[o]) * mk + fix
+         + g1 * m9r * hmax * (ddiv(u * (R)0.5 * (U4[IX(i + 1, j)] - U4[IX(i - 1, j)]), dx)
+                              + ddiv(v * (R)0.5 * (U4[IX(i, j + 1)] - U4[IX(i, j - 1)]), dy)
+                              + ddiv(U4[o] * (R)0.5 * (U[IX(i + 1, j)] - U[IX(i - 1, j)]), dx)
+                              + ddiv(V4[o] * (R)0.5 * (U[IX(i, j + 1)] - U[IX(i, j - 1)]), dy)
+                              - g2 * m9r * (d.U1pp[o] + d.U2[o] + d.U3[o]))
+         + g1 * m9r * divpq * (U4[o] - d.U1p[o]) + wm * u;
+  if (p.FRIC_SPONGE) sx = sx - d.CD_4_SPONGE[o] * u * spd * d.Depth[o];
+  R fiy = -(d.Cd[o] * v * spd);
+  d.FrcInsY[o] = fiy;
+  R sy = ddiv(GRAV * (d.Eta[o]), dy) * (d.DepthY[IX(i, j + 1)] - d.DepthY[o]) * mk + fiy
+         + g1 * m9r * hmax * (ddiv(u * (R)0.5 * (V4[IX(i + 1, j)] - V4[IX(i - 1, j)]), dx)
+                              + ddiv(v * (R)0.5 * (V4[IX(i, j + 1)] - V4[IX(i, j - 1)]), dy)
+                              + ddiv(U4[o] * (R)0.5 * (V[IX(i + 1, j)] - V[IX(i - 1, j)]), dx)
+                              + ddiv(V4[o] * (R)0.5 * (V[IX(i, j + 1)] - V[IX(i, j - 1)]), dy)
+                              - g2 * m9r * (d.V1pp[o] + d.V2[o] + d.V3[o]))
+         + g1 * m9r * divpq * (V4[o] - d.V1p[o]) + wm * v;
+  if (p.FRIC_SPONGE) sy = sy - d.CD_4_SPONGE[o] * v * spd * d.Depth[o];
+  if (p.VISC_BRK || p.DIFF_SPONGE) {
+    auto nuv = [&](int a, int b) -> float {
+      size_t q = IX(a, b);
+      float x = p.VISC_BRK ? (float)d.nu_break[q] : 0.0f;
+      if (p.DIFF_SPONGE) x = (float)((R)x + d.nu_sponge[q]);
+      return x;
+    };
+    float n0 = nuv(i, j);
+    R sxp = (R)(nuv(i + 1, j) + n0), sxm = (R)(nuv(i - 1, j) + n0);
+    R syp = (R)(nuv(i, j + 1) + n0), sym = (R)(nuv(i, j - 1) + n0);
+    const R *HU = d.HU, *HV = d.HV;
+    R h5x = ddiv((R)0.5, dx), h5y = ddiv((R)0.5, dy);
+    R bx = h5x * (ddiv(sxp * (R)1, dx) * (HU[IX(i + 1, j)] - HU[o]) - ddiv(sxm * (R)1, dx) * (HU[o] - HU[IX(i - 1, j)]))
+           + h5y * (ddiv(syp * (R)1, dy) * (HU[IX(i, j + 1)] - HU[o]) - ddiv(sym * (R)1, dy) * (HU[o] - HU[IX(i, j - 1)]));
+    d.BreakSourceX[o] = bx;
+    sx = sx + bx;
+    R by = h5x * (ddiv(sxp * (R)1, dx) * (HV[IX(i + 1, j)] - HV[o]) - ddiv(sxm * (R)1, dx) * (HV[o] - HV[IX(i - 1, j)]))
+           + h5y * (ddiv(syp * (R)1, dy) * (HV[IX(i, j + 1)] - HV[o]) - ddiv(sym * (R)1, dy) * (HV[o] - HV[IX(i, j - 1)]));
+    d.BreakSourceY[o] = by;
+    sy = sy + by;
+  }
+  d.SourceX[o] = sx;
+  d.SourceY[o] = sy;
+}
+
+// --------------------------------------------------------------------------
+// ESTIMATE_HUV (etauv_solver.F:53-351): SSP-RK3 update of Eta, Ubar, Vbar;
+// then H = Eta*Gamma3 + Depth on all cells (GET_Eta_U_V_HU_HV :373) and the
+// normalised tridiagonal rows of both line systems (:400-539).
+template <class R>
+__global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.M || j > g.N) return;
+  size_t o = IX(i, j);
+  const bool inner = (i >= g.Ib && i <= g.Ie && j >= g.Jb && j <= g.Je);
+  const R mdf = p.MDF;
+  R eta = d.Eta[o];
+  if (inner) {
+    const R *P = d.P, *Q = d.Q;
+    R hcur = d.H[o];
+    R pc = (R)0.5 * (P[IX(i + 1, j)] + P[o]);
+    R qc = (R)0.5 * (Q[IX(i, j + 1)] + Q[o]);
+    d.P_center[o] = pc;
+    d.Q_center[o] = qc;
+    d.U_davg[o] = ddiv(pc, rmax(hcur, mdf));
+    d.V_davg[o] = ddiv(qc, rmax(hcur, mdf));
+    const R nl = (R)-1, nr = (R)1;
+    R fl = P[o], fr = P[IX(i + 1, j)], fb = Q[o], ft = Q[IX(i, j + 1)];
+    R r1 = -(ddiv((R)1, p.DXg) * (fr * nr + fl * nl)) - ddiv((R)1, p.DYg) * (ft * nr + fb * nl);
+    if (p.wavemaker != 0) {
+      R xmk = (R)(i - g.Ib) * p.DXg, ymk = (R)(j - g.Jb) * p.DYg;
+      if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < ddiv(p.YW, (R)2)) r1 = r1 + d.WaveMaker_Mass[o];
+    }
+    d.Wsurf[o] = r1;
+    eta = al * d.Eta0[o] + be * (eta + DT * r1);
+    if (p.ETA_LIMITER) {
+      if (eta < p.TROUGH) eta = p.TROUGH;
+      if (eta > p.CREST) eta = p.CREST;
+    }
+    d.Eta[o] = eta;
+    fl = d.Fx[o]; fr = d.Fx[IX(i + 1, j)]; fb = d.Fy[o]; ft = d.Fy[IX(i, j + 1)];
+    R r2 = -(ddiv((R)1, p.DX) * (fr * nr + fl * nl)) - ddiv((R)1, p.DY) * (ft * nr + fb * nl) + d.SourceX[o];
+    R ub = al * d.Ubar0[o] + be * (d.Ubar[o] + DT * r2);
+    d.Ubar[o] = ub;
+    fl = d.Gx[o]; fr = d.Gx[IX(i + 1, j)]; fb = d.Gy[o]; ft = d.Gy[IX(i, j + 1)];
+    R r3 = -(ddiv((R)1, p.DX) * (fr * nr + fl * nl)) - ddiv((R)1, p.DY) * (ft * nr + fb * nl) + d.SourceY[o];
+    R vb = al * d.Vbar0[o] + be * (d.Vbar[o] + DT * r3);
+    d.Vbar[o] = vb;
+    R h = eta * p.G3 + d.Depth[o];
+    d.H[o] = h;
+    if (p.DISPERSION) {
+      const R dx = p.DX, dy = p.DY, b1 = p.b1, b2 = p.b2;
+      const int mk = d.MASK[o];
+      const R gm = p.G1 * (R)d.MASK9[o];
+      R dep = rmax(d.Depth[o], mdf);
+      {
+        R depl = rmax(d.Depth[IX(i - 1, j)], mdf), depr = rmax(d.Depth[IX(i + 1, j)], mdf);
+        R t1 = gm * (ddiv(ddiv(ddiv(b1, (R)2), dx), dx) * dep * dep + ddiv(ddiv(b2, dx), dx) * depl * dep);
+        R t2 = (R)1 + gm * (-(ddiv(ddiv(b1, dx), dx) * dep * dep) - ddiv(ddiv((R)2 * b2, dx), dx) * dep * dep);
+        R t3 = gm * (ddiv(ddiv(ddiv(b1, (R)2), dx), dx) * dep * dep + ddiv(ddiv(b2, dx), dx) * dep * depr);
+        R t4 = ddiv(ub * (R)mk, rmax(h, mdf)) + gm * (-(ddiv(b1, (R)2) * dep * dep * d.Vxy[o]) - b2 * dep * d.DVxy[o]);
+        if (t2 != (R)0 || mk > 0) {
+          d.xa[o] = ddiv(t1, t2); d.xc[o] = ddiv(t3, t2); d.xd[o] = ddiv(t4, t2);
+        } else {
+          d.xa[o] = (R)0; d.xc[o] = (R)0; d.xd[o] = (R)0;
+        }
+      }
+      {
+        R depl = rmax(d.Depth[IX(i, j - 1)], mdf), depr = rmax(d.Depth[IX(i, j + 1)], mdf);
+        R t1 = gm * (ddiv(ddiv(ddiv(b1, (R)2), dy), dy) * dep * dep + ddiv(ddiv(b2, dy), dy) * depl * dep);
+        R t2 = (R)1 + gm * (-(ddiv(ddiv(b1, dy), dy) * dep * dep) - ddiv(ddiv((R)2 * b2, dy), dy) * dep * dep);
+        R t3 = gm * (ddiv(ddiv(ddiv(b1, (R)2), dy), dy) * dep * dep + ddiv(ddiv(b2, dy), dy) * dep * depr);
+        R t4 = ddiv(vb * (R)mk, rmax(h, mdf)) + gm * (-(ddiv(b1, (R)2) * dep * dep * d.Uxy[o]) - b2 * dep * d.DUxy[o]);
+        if (t2 != (R)0 || mk > 0) {
+          d.ya[o] = ddiv(t1, t2); d.yc[o] = ddiv(t3, t2); d.yd[o] = ddiv(t4, t2);
+        } else {
+          d.ya[o] = (R)0; d.yc[o] = (R)0; d.yd[o] = (R)0;
+        }
+      }
+    } else {
+      d.U[o] = ddiv(ub, rmax(h, mdf));
+      d.V[o] = ddiv(vb, rmax(h, mdf));
+    }
+  } else {
+    d.H[o] = eta * p.G3 + d.Depth[o];
+  }
+}
+
+// TRIDx_ser / TRIDy_ser (tridiagnal.F:338-398): Thomas sweeps on rows
+// normalised by the diagonal; a row with A == 0 is left as it is.  One
+// thread per line; x lines walk along the contiguous direction, y lines are
+// coalesced across threads.
+template <class R>
+__device__ __forceinline__ void thomas(R *A, R *C, R *D, R *Z, size_t o0, ptrdiff_t s, int n) {
+  R cprev = C[o0], dprev = D[o0];
+  for (int k = 1; k < n; k++) {
+    size_t o = o0 + k * s;
+    R a = A[o];
+    R c = C[o], dd = D[o];
+    if (a != (R)0) {
+      R den = ddiv((R)1, a) - cprev;
+      c = ddiv(ddiv(c, a), den);
+      dd = ddiv(ddiv(dd, a) - dprev, ddiv((R)1, a) - cprev);
+      C[o] = c;
+      D[o] = dd;
+    }
+    cprev = c;
+    dprev = dd;
+  }
+  size_t ol = o0 + (n - 1) * s;
+  R z = D[ol];
+  Z[ol] = z;
+  for (int k = n - 2; k >= 0; k--) {
+    size_t o = o0 + k * s;
+    z = D[o] - C[o] * z;
+    Z[o] = z;
+  }
+}
+
+template <class R>
+__global__ void k_trid(Dev<R> d, Geo g, Prm<R> p, int nxblk) {
+  const int nx = g.Ie - g.Ib + 1, ny = g.Je - g.Jb + 1;
+  if ((int)blockIdx.x < nxblk) {
+    int j = g.Jb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > g.Je) return;
+    thomas(d.xa, d.xc, d.xd, d.U, IX(g.Ib, j), 1, nx);
+    return;
+  }
+  int i = g.Ib + (blockIdx.x - nxblk) * blockDim.x + threadIdx.x;
+  if (i > g.Ie) return;
+  const ptrdiff_t s = (ptrdiff_t)g.P;
+  if (!p.PERIODIC) {
+    thomas(d.ya, d.yc, d.yd, d.V, IX(i, g.Jb), s, ny);
+    return;
+  }
+  // TRIDy_periodic (tridiagnal.F:59-325, serial path): Sherman-Morrison
+  R *a = d.ya, *c = d.yc, *dd = d.yd;
+  size_t ob = IX(i, g.Jb), oe = IX(i, g.Je);
+  R abeg = a[ob], cend = c[oe];
+  c[ob] = ddiv(c[ob], (R)1 + cend);
+  dd[ob] = ddiv(dd[ob], (R)1 + cend);
+  a[oe] = ddiv(a[oe], (R)1 + abeg);
+  dd[oe] = ddiv(dd[oe], (R)1 + abeg);
+  for (int k = 0; k < ny; k++) d.c1[ob + k * s] = c[ob + k * s];
+  thomas(a, d.c1, dd, d.y1, ob, s, ny);
+  for (int k = 0; k < ny; k++) dd[ob + k * s] = (R)0;
+  dd[ob] = ddiv((R)1, (R)1 + cend);
+  dd[oe] = -ddiv((R)1, (R)1 + abeg);
+  thomas(a, c, dd, d.y2, ob, s, ny);
+  R beta = ddiv(cend * d.y1[ob] - abeg * d.y1[oe], (R)1 - (cend * d.y2[ob] - abeg * d.y2[oe]));
+  for (int k = 0; k < ny; k++) {
+    size_t o = ob + k * s;
+    d.V[o] = d.y1[o] + beta * d.y2[o];
+  }
+}
+
+// GET_Eta_U_V_HU_HV tail (etauv_solver.F:568-593): dry cells, HU/HV, Froude
+// cap; without dispersion Ubar/Vbar := HU/HV on all cells.
+template <class R>
+__global__ void k_uv_final(Dev<R> d, Geo g, Prm<R> p) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.M || j > g.N) return;
+  size_t o = IX(i, j);
+  if (i >= g.Ib && i <= g.Ie && j >= g.Jb && j <= g.Je) {
+    if (d.MASK[o] < 1) {
+      d.Ubar[o] = (R)0; d.Vbar[o] = (R)0; d.U[o] = (R)0; d.V[o] = (R)0; d.HU[o] = (R)0; d.HV[o] = (R)0;
+    } else {
+      R hm = rmax(d.H[o], p.MDF);
+      R u = d.U[o], v = d.V[o];
+      R hu = hm * u, hv = hm * v;
+      R ut = dsqrt(u * u + v * v);
+      R fr = dsqrt(GRAV * hm);
+      if (ddiv(ut, fr) > p.FCAP) {
+        R th = datan2(v, u);
+        u = p.FCAP * fr * dcos(th);
+        v = p.FCAP * fr * dsin(th);
+        hu = u * hm;
+        hv = v * hm;
+        d.U[o] = u;
+        d.V[o] = v;
+      }
+      d.HU[o] = hu;
+      d.HV[o] = hv;
+    }
+  }
+}
+template <class R>
+__global__ void k_ubar_from_hu(Dev<R> d, size_t n) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    d.Ubar[q] = d.HU[q];
+    d.Vbar[q] = d.HV[q];
+  }
+}
+
+// --------------------------------------------------------------------------
+// UPDATE_MASK (masks.F:51-250).  Reads the old MASK, writes the new one to
+// MASKn (host swaps the buffers), truncates DepthX/DepthY at faces of cells
+// that were dry (later cell in loop order wins), and sets MASK9.
+template <class R>
+__device__ __forceinline__ int new_mask(const Dev<R> &d, const Geo &g, int i, int j) {
+  size_t o = IX(i, j);
+  int m = d.MASK[o];
+  if (i < g.Ib - 2 || i > g.Ie + 2 || j < g.Jb - 2 || j > g.Je + 2) return m;
+  if (d.MASK_STRUC[o] != 1) return m;
+  R e = d.Eta[o];
+  if (m < 1) {
+    if (e > -d.Depth[o]) m = 1;
+    if (d.MASK[IX(i - 1, j)] == 1 && d.Eta[IX(i - 1, j)] > e) m = 1;
+    if (d.MASK[IX(i + 1, j)] == 1 && d.Eta[IX(i + 1, j)] > e) m = 1;
+    if (d.MASK[IX(i, j - 1)] == 1 && d.Eta[IX(i, j - 1)] > e) m = 1;
+    if (d.MASK[IX(i, j + 1)] == 1 && d.Eta[IX(i, j + 1)] > e) m = 1;
+  } else {
+    if (e < -d.Depth[o]) m = 0;
+  }
+  return m;
+}
+
+template <class R>
+__global__ void k_mask(Dev<R> d, Geo g, Prm<R> p) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.M + 1 || j > g.N + 1) return;
+  const int lo_i = g.Ib - 2, hi_i = g.Ie + 2, lo_j = g.Jb - 2, hi_j = g.Je + 2;
+  // x face (i, j): written by cells i-1 (-> Depth(i)) and i (-> Depth(i-1))
+  if (j <= g.N && j >= lo_j && j <= hi_j) {
+    R v;
+    bool w = false;
+    if (i - 1 >= lo_i && i - 1 <= hi_i && d.MASK[IX(i - 1, j)] < 1) { v = d.Depth[IX(i, j)]; w = true; }
+    if (i >= lo_i && i <= hi_i && d.MASK[IX(i, j)] < 1) { v = d.Depth[IX(i - 1, j)]; w = true; }
+    if (w) d.DepthX[IX(i, j)] = v;
+  }
+  if (i <= g.M && i >= lo_i && i <= hi_i) {
+    R v;
+    bool w = false;
+    if (j - 1 >= lo_j && j - 1 <= hi_j && d.MASK[IX(i, j - 1)] < 1) { v = d.Depth[IX(i, j)]; w = true; }
+    if (j >= lo_j && j <= hi_j && d.MASK[IX(i, j)] < 1) { v = d.Depth[IX(i, j - 1)]; w = true; }
+    if (w) d.DepthY[IX(i, j)] = v;
+  }
+  if (i > g.M || j > g.N) return;
+  size_t o = IX(i, j);
+  d.MASKn[o] = new_mask(d, g, i, j);
+  int m9 = 0;
+  if (i >= g.Ib - 1 && i <= g.Ie + 1 && j >= g.Jb - 1 && j <= g.Je + 1) {
+    if (p.VISC_BRK) {
+      m9 = 1;
+    } else {
+      m9 = new_mask(d, g, i, j) * new_mask(d, g, i - 1, j) * new_mask(d, g, i + 1, j) * new_mask(d, g, i + 1, j + 1) *
+           new_mask(d, g, i, j + 1) * new_mask(d, g, i - 1, j + 1) * new_mask(d, g, i + 1, j - 1) *
+           new_mask(d, g, i, j - 1) * new_mask(d, g, i - 1, j - 1);
+      if (ddiv(dabs(d.Eta[o]), rmax(d.Depth[o], p.MDF)) > p.SWE) m9 = 0;
+    }
+  }
+  d.MASK9[o] = m9;
+}
+
+// --------------------------------------------------------------------------
+// BREAKING (breaker.F:87-353).  The reference updates AGE in place in loop
+// order, so a cell reads the new age of its neighbours (I-1,J), (I-1,J-1),
+// (I,J-1), (I+1,J-1) and the old age of the others.  Pass 1 evaluates every
+// cell with old neighbour ages and lists the cells whose result depends on a
+// "new" neighbour; pass 2 (one workgroup) iterates those to the fixed point,
+// which is the in-order result because the dependencies form a DAG.
+template <class R>
+struct BrkIn {
+  R t1, t2, t3, ang, et, c, px, py, pxy;
+};
+template <class R>
+__device__ __forceinline__ BrkIn<R> brk_in(const Dev<R> &d, const Geo &g, const Prm<R> &p, size_t o) {
+  BrkIn<R> b;
+  R h = d.H[o];
+  b.t3 = dsqrt(GRAV * rmax(p.MDF, h));
+  b.t1 = p.C1 * b.t3;
+  b.t2 = p.C2 * b.t3;
+  R ex = d.ETAx[o], ey = d.ETAy[o];
+  b.ang = ddiv(datan2(-ey, -ex) * (R)180, FW_PI);
+  b.et = d.ETAT[o];
+  R t4 = rmax(dsqrt(ex * ex + ey * ey), SMALL);
+  b.c = rmin(ddiv(dabs(b.et), t4), dsqrt(GRAV * dabs(h)));
+  b.pxy = ddiv(dsqrt(p.DX * p.DX + p.DY * p.DY), rmax(b.c, SMALL));
+  b.px = ddiv(dsqrt(p.DX * p.DX), rmax(b.c, SMALL));
+  b.py = ddiv(dsqrt(p.DY * p.DY), rmax(b.c, SMALL));
+  return b;
+}
+// quadrant of the propagation test: 0 none, 1 [0,90), 2 [90,180),
+// 3 [-180,-90) (reads only old ages), 4 [-90,0)
+template <class R>
+__device__ __forceinline__ int brk_quad(R ang) {
+  if (ang >= (R)0 && ang < (R)90) return 1;
+  if (ang >= (R)90 && ang < (R)180) return 2;
+  if (ang >= (R)-180 && ang < (R)-90) return 3;
+  if (ang >= (R)-90 && ang < (R)0) return 4;
+  return 0;
+}
+template <class R>
+__device__ __forceinline__ bool brk_trig(R a1, R p1, R a2, R p2, R a3, R p3, R DT) {
+  return (a1 >= DT && a1 > p1) || (a2 >= DT && a2 > p2) || (a3 >= DT && a3 > p3);
+}
+// new age of a cell in the propagation branch; nw/old give neighbour ages
+template <class R>
+__device__ __forceinline__ R brk_prop(const Dev<R> &d, const Geo &g, const BrkIn<R> &b, int q, int i, int j, const R *nw,
+                                      const R *old, R DT, R own) {
+  bool t = false;
+  if (q == 1) t = brk_trig(nw[IX(i - 1, j)], b.px, nw[IX(i - 1, j - 1)], b.pxy, nw[IX(i, j - 1)], b.py, DT);
+  if (q == 2) t = brk_trig(old[IX(i + 1, j)], b.px, nw[IX(i + 1, j - 1)], b.pxy, nw[IX(i, j - 1)], b.py, DT);
+  if (q == 3) t = brk_trig(old[IX(i + 1, j)], b.px, old[IX(i + 1, j + 1)], b.pxy, old[IX(i, j + 1)], b.py, DT);
+  if (q == 4) t = brk_trig(old[IX(i, j + 1)], b.py, old[IX(i - 1, j + 1)], b.pxy, nw[IX(i - 1, j)], b.px, DT);
+  return t ? DT : own;
+}
+template <class R>
+__device__ __forceinline__ R brk_nu(const Dev<R> &d, const Geo &g, const Prm<R> &p, const BrkIn<R> &b, size_t o, int i,
+                                    int j, R age) {
+  R xmk = (R)(i - g.Ib) * p.DX, ymk = (R)(j - g.Jb) * p.DY;
+  if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < ddiv(p.YW, (R)2)) {
+    if (b.et > rmin(b.t2, p.WKC * b.t3)) {
+      R cap1 = (R)1 * (rmax(d.Depth[o], p.MDF) + d.Eta[o]);
+      return cap1 * p.WKC * b.t3 + p.NUB;
+    }
+    return (R)0 + p.NUB;
+  }
+  if (age > (R)0 && age < p.TBRK && b.et > b.t2) {
+    R cap1 = (R)1 * (rmax(d.Depth[o], p.MDF) + d.Eta[o]);
+    return cap1 * b.t2 + p.NUB;
+  }
+  return (R)0 + p.NUB;
+}
+
+template <class R>
+__global__ void k_brk1(Dev<R> d, Geo g, Prm<R> p, R DT) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.M || j > g.N) return;
+  size_t o = IX(i, j);
+  R age = d.AGE_BREAKING[o];
+  if (i < g.Ib - 1 || i > g.Ie + 1 || j < g.Jb - 1 || j > g.Je + 1) {
+    d.AGEn[o] = age;
+    return;
+  }
+  BrkIn<R> b = brk_in(d, g, p, o);
+  R na;
+  if (b.et >= b.t1 && (age == (R)0 || age > p.TBRK)) {
+    na = DT;
+  } else if (age > (R)0) {
+    na = age + DT;
+  } else {
+    na = age;
+    if (b.et >= b.t2) {
+      int q = brk_quad(b.ang);
+      if (q != 0) {
+        na = brk_prop(d, g, b, q, i, j, d.AGE_BREAKING, d.AGE_BREAKING, DT, age);
+        if (q != 3) {
+          int k = atomicAdd(d.cand_n, 1);
+          d.cand[k] = (int)o;
+        }
+      }
+    }
+  }
+  d.AGEn[o] = na;
+  d.nu_break[o] = brk_nu(d, g, p, b, o, i, j, na);
+}
+
+template <class R>
+__global__ void __launch_bounds__(1024) k_brk2(Dev<R> d, Geo g, Prm<R> p, R DT) {
+  __shared__ int changed;
+  const int n = *d.cand_n;
+  if (n == 0) return;
+  // the dependency chains are at most n long, so n+1 sweeps reach the fixed point
+  const int maxit = n + 2;
+  int it = 0;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) changed = 0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      size_t o = (size_t)d.cand[k];
+      int i = 1 + (int)(o % g.P), j = 1 + (int)(o / g.P);
+      BrkIn<R> b = brk_in(d, g, p, o);
+      R na = brk_prop(d, g, b, brk_quad(b.ang), i, j, d.AGEn, d.AGE_BREAKING, DT, d.AGE_BREAKING[o]);
+      if (na != d.AGEn[o]) {
+        d.AGEn[o] = na;
+        changed = 1;
+      }
+    }
+    __syncthreads();
+    if (!changed) break;
+    if (++it >= maxit) {
+      if (threadIdx.x == 0) d.flags[0] = 1;
+      break;
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    size_t o = (size_t)d.cand[k];
+    int i = 1 + (int)(o % g.P), j = 1 + (int)(o / g.P);
+    BrkIn<R> b = brk_in(d, g, p, o);
+    d.nu_break[o] = brk_nu(d, g, p, b, o, i, j, d.AGEn[o]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *d.cand_n = 0;
+}
+
+// --------------------------------------------------------------------------
+// EXCHANGE tail (bc.F:491-494): U,V,HU,HV *= MASK on all cells, optionally
+// followed by SPONGE_DAMPING (sponge.F:52-66).
+template <class R>
+__global__ void k_mask_mul(Dev<R> d, Geo g, Prm<R> p, int sponge, int mul) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.M || j > g.N) return;
+  size_t o = IX(i, j);
+  int mk = d.MASK[o];
+  R u = d.U[o], v = d.V[o];
+  if (mul) {
+    R m = (R)mk;
+    u = u * m;
+    v = v * m;
+    d.HU[o] = d.HU[o] * m;
+    d.HV[o] = d.HV[o] * m;
+  }
+  if (sponge) {
+    R s = d.SPONGE[o];
+    if ((R)mk > (R)0) d.Eta[o] = ddiv(d.Eta[o], s);
+    u = ddiv(u, s);
+    v = ddiv(v, s);
+  }
+  d.U[o] = u;
+  d.V[o] = v;
+}
+
+// FP64 budget (mass, energy, extrema) over the interior.
+template <class R>
+__global__ void k_stats(Dev<R> d, Geo g, Prm<R> p) {
+  __shared__ double sm[7][256];
+  double mass = 0, en = 0, mx = -1e300, mn = 1e300, mu = 0, mv = 0, wet = 0;
+  int nx = g.Ie - g.Ib + 1, n = nx * (g.Je - g.Jb + 1);
+  double dxdy = (double)p.DX * (double)p.DY;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    int i = g.Ib + q % nx, j = g.Jb + q / nx;
+    size_t o = IX(i, j);
+    if (d.MASK[o] < 1) continue;
+    wet += 1.0;
+    double e = (double)d.Eta[o], u = (double)d.U[o], v = (double)d.V[o];
+    double h = e + (double)d.Depth[o];  // total depth formed in FP64
+    mass += e * dxdy;
+    en += (0.5 * h * (u * u + v * v) + 0.5 * 9.81 * e * e) * dxdy;
+    mx = e > mx ? e : mx;
+    mn = e < mn ? e : mn;
+    mu = fabs(u) > mu ? fabs(u) : mu;
+    mv = fabs(v) > mv ? fabs(v) : mv;
+  }
+  int t = threadIdx.x;
+  sm[0][t] = mass; sm[1][t] = en; sm[2][t] = mx; sm[3][t] = mn; sm[4][t] = mu; sm[5][t] = mv; sm[6][t] = wet;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      sm[0][t] += sm[0][t + s];
+      sm[1][t] += sm[1][t + s];
+      sm[6][t] += sm[6][t + s];
+      sm[2][t] = sm[2][t] > sm[2][t + s] ? sm[2][t] : sm[2][t + s];
+      sm[3][t] = sm[3][t] < sm[3][t + s] ? sm[3][t] : sm[3][t + s];
+      sm[4][t] = sm[4][t] > sm[4][t + s] ? sm[4][t] : sm[4][t + s];
+      sm[5][t] = sm[5][t] > sm[5][t + s] ? sm[5][t] : sm[5][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int k = 0; k < 7; k++) d.red64[blockIdx.x * 7 + k] = sm[k][0];
+}
+
+#undef IX
+}  // namespace fw

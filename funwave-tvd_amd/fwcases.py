@@ -53,8 +53,8 @@ class Case:
         os.makedirs(path, exist_ok=True)
         if self.depth is not None:
             with open(os.path.join(path, "depth.txt"), "w") as f:
-                for row in self.depth:
-                    f.write(" ".join(f"{x:.9e}" for x in row) + "\n")
+                # 17 significant digits: the text round-trips the double exactly
+                np.savetxt(f, np.asarray(self.depth, np.float64), fmt="%.16e")
         if self.spectrum is not None:
             s = self.spectrum
             with open(os.path.join(path, "spectrum.txt"), "w") as f:
@@ -232,7 +232,7 @@ def member(seed, dx=2.0, mrange=(256, 1024), nrange=(128, 768),
 
 
 def island(seed, dx=4.0, mrange=(256, 768), nrange=(128, 512),
-           total_time=1.0e9, mglob=None, nglob=None):
+           total_time=1.0e9, mglob=None, nglob=None, r_base=None, h0=None):
     """Solitary wave (INI_SOL, src/init.F:907-910) running up a seeded
     conical island (slope 1:4, base radius ~U[50,200] m, 5 % noise)."""
     rng = np.random.default_rng(9000 + seed)
@@ -240,18 +240,19 @@ def island(seed, dx=4.0, mrange=(256, 768), nrange=(128, 512),
         mglob = int(rng.integers(mrange[0], mrange[1] + 1))
     if nglob is None:
         nglob = int(rng.integers(nrange[0], nrange[1] + 1))
-    h0 = float(rng.uniform(0.32, 10.0))
+    h0_draw = float(rng.uniform(0.32, 10.0))
     ah = float(rng.uniform(0.05, 0.3))
-    r_base = float(rng.uniform(50.0, 200.0))
+    r_draw = float(rng.uniform(50.0, 200.0))
+    h0 = h0_draw if h0 is None else float(h0)
+    r_base = r_draw if r_base is None else float(r_base)
     lx, ly = mglob * dx, nglob * dx
     x = (np.arange(mglob) + 0.5) * dx
     y = (np.arange(nglob) + 0.5) * dx
     X, Y = np.meshgrid(x, y)
     cx, cy = 0.6 * lx, 0.5 * ly
     r = np.sqrt((X - cx) ** 2 + (Y - cy) ** 2)
-    top = h0 + 2.0  # cone rises 2 m above still water
-    cone = np.clip(top - (r - (r_base - 4.0 * top)) / 4.0, 0.0, top)
-    cone = np.where(r < r_base, np.minimum(cone, (r_base - r) / 4.0), 0.0)
+    top = h0 + 0.5 * h0  # the cone rises above still water by h0/2
+    cone = np.clip((r_base - r) / 4.0, 0.0, top)
     cone = cone * (1.0 + 0.05 * rng.standard_normal(cone.shape))
     depth = h0 - cone
     p = _common(mglob, nglob, dx, dx, total_time)

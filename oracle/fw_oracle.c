@@ -763,7 +763,7 @@ void fwo_sources(const fwo_params *p, fwo_fields *f, double time_) {
   if (p->VISCOSITY_BREAKING)
     for (size_t q = 0; q < ncell; q++) nuv[q] = (float)F(nu_break)[q];
   if (p->DIFFUSION_SPONGE)
-    for (size_t q = 0; q < ncell; q++) nuv[q] = nuv[q] + (float)F(nu_sponge)[q];
+    for (size_t q = 0; q < ncell; q++) nuv[q] = (float)((R)nuv[q] + F(nu_sponge)[q]);
 #define NUV(i, j) nuv[((size_t)(j)-1) * M + (size_t)(i)-1]
   R *U = F(U), *V = F(V), *U4 = F(U4), *V4 = F(V4), *Eta = F(Eta), *H = F(H);
   R *P = F(P), *Q = F(Q), *HU = F(HU), *HV = F(HV);
@@ -1190,4 +1190,48 @@ int fwo_run(const fwo_params *p, fwo_fields *f, int nsteps, double *dt, double *
     if (fwo_max_abs_eta(p, f) > p->EtaBlowVal) return 1;
   }
   return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* unit-level hooks (tests only) */
+void fwo_thomas_line(int n, void *a_, void *c_, void *d_, void *z_) {
+  R *a = (R *)a_, *c = (R *)c_, *d = (R *)d_, *z = (R *)z_;
+  for (int k = 1; k < n; k++) {
+    if (a[k] != ZERO) {
+      R den = SPC(1.0) / a[k] - c[k - 1];
+      c[k] = c[k] / a[k] / den;
+      d[k] = (d[k] / a[k] - d[k - 1]) / (SPC(1.0) / a[k] - c[k - 1]);
+    }
+  }
+  z[n - 1] = d[n - 1];
+  for (int k = n - 2; k >= 0; k--) z[k] = d[k] - c[k] * z[k + 1];
+}
+
+double fwo_hll(double sl, double sr, double fl, double fr, double ul, double ur) {
+  return (double)hll((R)sl, (R)sr, (R)fl, (R)fr, (R)ul, (R)ur);
+}
+
+void fwo_muscl_row(int n, const void *vin_, const int *mask, void *outl_, void *outr_) {
+  fwo_params p;
+  memset(&p, 0, sizeof p);
+  p.Mloc = n;
+  p.Nloc = 7;
+  p.Ibeg = 4;
+  p.Iend = n - 3;
+  p.Jbeg = 4;
+  p.Jend = 4;
+  size_t M = (size_t)n, M1 = M + 1;
+  R *vin = (R *)calloc(M * 7, sizeof(R));
+  int *mk = (int *)calloc(M * 7, sizeof(int));
+  R *ol = (R *)calloc(M1 * 7, sizeof(R)), *orr = (R *)calloc(M1 * 7, sizeof(R));
+  for (size_t i = 0; i < M; i++) {
+    vin[3 * M + i] = ((const R *)vin_)[i];
+    mk[3 * M + i] = mask[i];
+  }
+  construct_x(&p, mk, vin, ol, orr);
+  for (size_t i = 0; i < M1; i++) {
+    ((R *)outl_)[i] = ol[3 * M1 + i];
+    ((R *)outr_)[i] = orr[3 * M1 + i];
+  }
+  free(vin); free(mk); free(ol); free(orr);
 }

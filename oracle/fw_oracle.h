@@ -86,4 +86,21 @@ int  fwo_run(const fwo_params *p, fwo_fields *f, int nsteps, double *dt, double 
 #ifdef __cplusplus
 }
 #endif
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* Unit-level hooks on single building blocks (tests only). */
+/* Thomas sweep of tridiagnal.F:338-398 on one line of length n (a, c, d
+ * normalised rows; a[0] and c[n-1] unused); solution in z. a/c/d modified. */
+void fwo_thomas_line(int n, void *a, void *c, void *d, void *z);
+/* HLL flux of fluxes.F:171-194 */
+double fwo_hll(double sl, double sr, double fl, double fr, double ul, double ur);
+/* CONSTRUCT_HO_X (fluxes.F:1531-1639) on one row of n cells (n >= 7, three
+ * ghost cells each side): face states OutL/OutR at faces 4..n-2 (1-based,
+ * face i between cells i-1 and i); mask entries int32. */
+void fwo_muscl_row(int n, const void *vin, const int *mask, void *outl, void *outr);
+#ifdef __cplusplus
+}
+#endif
+
 #endif

@@ -1,0 +1,43 @@
+"""Small workloads used by the parity tests (sizes the CPU checkers finish
+in seconds).  Each is a reduced version of a benchmark workload written in
+the reference's input format by funwave-tvd_amd/fwcases.py."""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(REPO, "funwave-tvd_amd"))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+
+import fwcases  # noqa: E402
+
+
+def tiny2d():
+    """2-D beach with wet/dry front, breaking, WK_DATA2D irregular waves and
+    sponges on three sides (reduced large-domain workload)."""
+    c = fwcases.beach_2d(mglob=64, nglob=48, dx=2.0, nfreq=4, ndir=3, flat_depth=2.0,
+                         slope=1 / 25.0, x_toe=60.0, xc_wk=20.0, sponge=0.0)
+    c.params["Sponge_west_width"] = 10.0
+    c.params["Sponge_south_width"] = 8.0
+    c.params["Sponge_north_width"] = 8.0
+    c.spectrum["amp"] = c.spectrum["amp"] * 4.0
+    c.name = "tiny2d"
+    return c
+
+
+def tiny1d():
+    """Quasi-1D periodic beach with the WK_REG source (reduced config 0)."""
+    c = fwcases.beach_1d(mglob=200, nglob=4, xslp=80, xc_wk=40, sponge_west=20,
+                         amp=0.5, period=5.0)
+    c.name = "tiny1d"
+    return c
+
+
+def island():
+    """Solitary wave running up a conical island (INI_SOL, no sponge)."""
+    c = fwcases.island(0, dx=4.0, mglob=72, nglob=48, r_base=60.0, h0=4.0)
+    c.name = "island"
+    return c
+
+
+CASES = {"tiny2d": tiny2d, "tiny1d": tiny1d, "island": island}

@@ -1,0 +1,65 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference
+(oracle/_ref/fwref_{sp,dp}, built by oracle/Makefile from /root/reference).
+
+For each small workload and precision: the full state after INITIALIZATION
+(every array that is not identically zero, plus the scalars) and the key
+state arrays after a number of whole time steps, all in full precision.
+Run from the repo root:  python tests/golden/make_golden.py
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import cases  # noqa: E402
+import oracle  # noqa: E402
+from fwdump import read_dump  # noqa: E402
+
+KEY = ["Eta", "U", "V", "Ubar", "Vbar", "HU", "HV", "H", "MASK", "MASK9",
+       "AGE_BREAKING", "nu_break", "DepthX", "DepthY", "P", "Q"]
+PLAN = {"tiny2d": (["dp", "sp"], [1, 60, 250]),
+        "tiny1d": (["dp", "sp"], [1, 400]),
+        "island": (["dp", "sp"], [1, 120])}
+
+
+def pack(d, keys=None):
+    out = {}
+    for k, v in d.items():
+        if keys is not None and k not in keys:
+            continue
+        if isinstance(v, np.ndarray):
+            if keys is None and not np.any(v):
+                continue
+            out[k] = v
+        elif isinstance(v, str):
+            out[k] = np.array(v)
+        else:
+            out[k] = np.array(v)
+    return out
+
+
+def main():
+    for name, (precs, steps) in PLAN.items():
+        for prec in precs:
+            with tempfile.TemporaryDirectory() as td:
+                c = cases.CASES[name]()
+                c.write(td)
+                last = max(steps)
+                oracle.run_reference(td, prec, last, prefix="g", dump_init=True, dump_every=1)
+                blob = {}
+                for k, v in pack(read_dump(os.path.join(td, "g_init.bin"))).items():
+                    blob[f"init/{k}"] = v
+                for s in steps:
+                    d = read_dump(os.path.join(td, f"g_step{s:06d}.bin"))
+                    for k, v in pack(d, set(KEY) | {"DT", "TIME"}).items():
+                        blob[f"s{s}/{k}"] = v
+                path = os.path.join(HERE, f"{name}_{prec}.npz")
+                np.savez_compressed(path, **blob)
+                print(path, os.path.getsize(path))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,145 @@
+"""GPU (libfwgpu.so through the C ABI) against the CPU oracle and the
+reference's golden runs.
+
+Per routine: starting from the same state, every routine of three whole
+steps is run on the GPU and in the C restatement; every array the step
+touches must agree bit for bit.  Whole runs: fw_step over hundreds of steps
+must reproduce the compiled reference's state bit for bit (fixtures)."""
+import numpy as np
+import pytest
+
+from common import STATE_KEYS, bitwise_mismatch, load_golden, oracle_available, rel_l2
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not oracle_available(), reason="oracle not built")]
+
+CASES = ["tiny2d", "tiny1d", "island"]
+
+
+def _solvers(name, prec):
+    import fwgpu
+    import oracle
+    init, steps = load_golden(name, prec)
+    return fwgpu.Solver(init, prec), oracle.OracleSolver(init), init, steps
+
+
+def _compare(g, o, where):
+    import fwgpu
+    bad = {}
+    for k in fwgpu.ALL_FIELDS:
+        if k in fwgpu.AUX:
+            continue
+        n = bitwise_mismatch(g.download(k), o[k])
+        if n:
+            bad[k] = n
+    assert not bad, f"{where}: {bad}"
+
+
+@pytest.mark.parametrize("prec", ["sp", "dp"])
+@pytest.mark.parametrize("name", CASES)
+def test_each_routine_matches_oracle(name, prec, gpu_available):
+    print(f"\n[routine parity] case={name} prec={prec}", flush=True)
+    g, o, init, _ = _solvers(name, prec)
+    _compare(g, o, "upload")
+    for step in range(1, 4):
+        g.save_stage0(1)
+        for a, b in (("Eta0", "Eta"), ("Ubar0", "Ubar"), ("Vbar0", "Vbar")):
+            o[a][...] = o[b]
+        g.exchange(); o.exchange(); _compare(g, o, f"s{step} exchange")
+        gdt, gt = g.estimate_dt(); o.estimate_dt()
+        assert (gdt, gt) == (o.dt.value, o.time.value)
+        g.save_stage0(2)
+        o["U0"][...] = o["U"]; o["V0"][...] = o["V"]
+        for st in (1, 2, 3):
+            if init["DISPERSION"]:
+                g.dispersion(); o.dispersion(); _compare(g, o, f"s{step}.{st} dispersion")
+            g.fluxes(); o.fluxes(); _compare(g, o, f"s{step}.{st} fluxes")
+            g.sources(); o.sources(); _compare(g, o, f"s{step}.{st} sources")
+            g.estimate_huv(st); o.estimate_huv(st); _compare(g, o, f"s{step}.{st} huv")
+            g.update_mask(); o.update_mask(); _compare(g, o, f"s{step}.{st} mask")
+            g.breaking(); o.breaking(); _compare(g, o, f"s{step}.{st} breaking")
+            g.exchange(); o.exchange(); _compare(g, o, f"s{step}.{st} exchange")
+            if init["DIRECT_SPONGE"]:
+                g.sponge(); o.sponge(); _compare(g, o, f"s{step}.{st} sponge")
+    g.close()
+
+
+PLAN = {"tiny2d": [1, 60, 250], "tiny1d": [1, 400], "island": [1, 120]}
+
+
+TOL = {"sp": 1e-5, "dp": 1e-10}
+
+
+@pytest.mark.parametrize("prec", ["sp", "dp"])
+@pytest.mark.parametrize("name", CASES)
+def test_whole_run_matches_reference(name, prec, gpu_available):
+    """fw_step over the fixture's steps against the compiled reference:
+    wet/dry masks identical at every checked step and eta/u/v relative L2
+    within 1e-5 (FP32) / 1e-10 (FP64).  The path is bitwise identical to the
+    reference except where the Froude cap or the breaking angle evaluates
+    ATAN2/COS/SIN per cell (device vs host math library, last-bit
+    differences); the bit-identical fraction is printed."""
+    import fwgpu
+    print(f"\n[whole-run parity] case={name} prec={prec}", flush=True)
+    init, steps = load_golden(name, prec)
+    g = fwgpu.Solver(init, prec)
+    done = 0
+    for s in PLAN[name]:
+        dt, t = g.step(s - done)
+        done = s
+        ref = steps[s]
+        assert dt == pytest.approx(ref["DT"], rel=100 * TOL[prec]), (s, dt, ref["DT"])
+        assert bitwise_mismatch(g.download("MASK"), ref["MASK"]) == 0, f"mask differs at step {s}"
+        errs = {k: rel_l2(g.download(k)[3:-3, 3:-3], ref[k][3:-3, 3:-3]) for k in ("Eta", "U", "V")}
+        nbit = {k: bitwise_mismatch(g.download(k), ref[k]) for k in STATE_KEYS}
+        print(f"  step {s}: rel L2 {errs}; arrays with non-identical bits: {dict((k, v) for k, v in nbit.items() if v)}")
+        assert all(e <= TOL[prec] for e in errs.values()), (s, errs)
+    g.close()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fp32_error_against_fp64_reference(name, gpu_available):
+    """The FP32 GPU state against the FP64 reference (the error stated for
+    every FP32 result): it must be no larger than the reference's own FP32
+    build's error against its FP64 build."""
+    import fwgpu
+    init32, steps32 = load_golden(name, "sp")
+    _, steps64 = load_golden(name, "dp")
+    last = max(PLAN[name])
+    g = fwgpu.Solver(init32, "sp")
+    g.step(last)
+    ref = steps64[last]
+    e = rel_l2(g.download("Eta")[3:-3, 3:-3], ref["Eta"][3:-3, 3:-3])
+    e_ref = rel_l2(steps32[last]["Eta"][3:-3, 3:-3], ref["Eta"][3:-3, 3:-3])
+    m = bitwise_mismatch(g.download("MASK"), ref["MASK"])
+    print(f"\n[fp32 vs fp64 reference] {name} after {last} steps: eta rel L2 GPU-FP32 = {e:.3e}, "
+          f"reference-FP32 = {e_ref:.3e}; mask cells differing from FP64 = {m}")
+    assert e <= e_ref * 1.05 + 1e-6
+    g.close()
+
+
+def test_blowup_is_reported(gpu_available):
+    import fwgpu
+    init, _ = load_golden("tiny2d", "sp")
+    init = dict(init)
+    init["EtaBlowVal"] = 1e-9
+    g = fwgpu.Solver(init, "sp")
+    with pytest.raises(fwgpu.FwError) as e:
+        g.step(3)
+    assert e.value.code == fwgpu.FW_E_BLOWUP
+    g.close()
+
+
+def test_fp64_budget_matches_host_sum(gpu_available):
+    import fwgpu
+    init, _ = load_golden("tiny2d", "sp")
+    g = fwgpu.Solver(init, "sp")
+    g.step(40)
+    s = g.stats()
+    eta = g.download("Eta")[3:-3, 3:-3].astype(np.float64)
+    mask = g.download("MASK")[3:-3, 3:-3]
+    dxdy = float(init["DX"]) * float(init["DY"])
+    mass = (eta * (mask > 0)).sum() * dxdy
+    assert s["wet_cells"] == int((mask > 0).sum())
+    assert s["mass_volume"] == pytest.approx(mass, rel=1e-12, abs=1e-9)
+    g.close()

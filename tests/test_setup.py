@@ -1,0 +1,50 @@
+"""The host-side initial state (funwave-tvd_amd/fwsetup.py, used by bench.py)
+against the compiled reference's INITIALIZATION (golden init fixtures)."""
+import numpy as np
+import pytest
+
+from common import load_golden
+
+
+@pytest.mark.parametrize("name", ["tiny2d", "tiny1d"])
+def test_setup_matches_reference_initialisation(name):
+    import cases
+    import fwsetup
+    ref, _ = load_golden(name, "dp")
+    st = fwsetup.build_state(cases.CASES[name](), "dp")
+    for k in ("Mloc", "Nloc", "Ibeg", "Iend", "Jbeg", "Jend", "ilo", "ihi", "jlo", "jhi",
+              "DISPERSION", "PERIODIC", "VISCOSITY_BREAKING", "SHOW_BREAKING", "DIRECT_SPONGE"):
+        assert int(st[k]) == int(ref[k]), k
+    for k in ("DX", "DY", "CFL", "MinDepth", "MinDepthFrc", "Gamma1", "Gamma2", "Gamma3",
+              "Beta_1", "Beta_2", "b1", "b2", "SWE_ETA_DEP", "FroudeCap", "Cbrk1", "Cbrk2",
+              "T_brk", "WAVEMAKER_Cbrk", "Xc_WK", "EtaBlowVal"):
+        assert st[k] == pytest.approx(ref[k], rel=1e-12, abs=1e-12), k
+    for k in ("Width_WK", "D_gen", "Beta_gen", "rlamda", "FreqPeak"):
+        assert st[k] == pytest.approx(ref[k], rel=1e-6, abs=1e-12), k
+    for k in ("Depth", "DepthX", "DepthY", "Eta", "H", "Cd"):
+        np.testing.assert_allclose(st[k], ref.get(k, 0 * st[k]), rtol=1e-12, atol=1e-12, err_msg=k)
+    np.testing.assert_array_equal(st["MASK"], ref["MASK"])
+    np.testing.assert_array_equal(st["MASK9"][3:-3, 3:-3], ref["MASK9"][3:-3, 3:-3])
+    np.testing.assert_allclose(st["SPONGE"], ref["SPONGE"], rtol=1e-6)
+    np.testing.assert_allclose(st["xmk_wk"][3:-3], ref["xmk_wk"][3:-3], rtol=1e-12)
+    if "Cm" in ref:
+        for k in ("Cm", "Sm"):
+            a, b = st[k], ref[k]
+            scale = np.abs(b).max()
+            band = (slice(None), slice(st["jlo"] - 1, st["jhi"]), slice(st["ilo"] - 1, st["ihi"]))
+            np.testing.assert_allclose(a[band], b[band], rtol=0, atol=1e-6 * scale, err_msg=k)
+        np.testing.assert_allclose(st["OMGN2D"], ref["OMGN2D"], rtol=1e-7)
+
+
+def test_setup_large_domain_is_sane():
+    """The 2048 x 2048 large-domain workload at a reduced size: a shoreline
+    exists, the wavemaker band lies offshore, sponges on W/S/N only."""
+    import fwcases
+    import fwsetup
+    c = fwcases.beach_2d(mglob=256, nglob=256, dx=16.0, nfreq=5, ndir=3)
+    st = fwsetup.build_state(c, "sp")
+    dry = 1.0 - st["MASK"][3:-3, 3:-3].mean()
+    assert 0.01 < dry < 0.2
+    assert st["ilo"] <= st["ihi"] and st["Xc_WK"] < 0.3 * 256 * 16.0
+    sp = st["SPONGE"]
+    assert sp[130, 3] > 1.0 and sp[3, 130] > 1.0 and sp[-4, 130] > 1.0 and sp[130, -4] == 1.0
