@@ -925,6 +925,140 @@ __global__ void k_trid(Dev<R> d, Geo g, Prm<R> p, int nxblk) {
   }
 }
 
+// Line solves staged through LDS.  A workgroup of 256 threads owns L lines
+// of one direction.  All threads load the next chunk of (a, c, d) with
+// coalesced accesses (consecutive threads -> consecutive addresses: along a
+// row for x lines, across columns for y lines) into a double-buffered LDS
+// tile while lanes 0..L-1 of wave 0 run the recurrence on the current chunk
+// out of LDS.  The forward sweep writes the modified (c', d') back in place
+// (every element is loaded one chunk before it is overwritten), the backward
+// sweep streams them back in reverse.  The arithmetic per element is the
+// reference's (tridiagnal.F:383-396), so results are bitwise the same as
+// the one-thread-per-line sweep.
+template <class R, int L, int CH>
+struct TridTile {
+  R a[2][L][CH + 1];
+  R c[2][L][CH + 1];
+  R d[2][L][CH + 1];
+};
+
+template <class R, int L, int CH>
+__global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
+  __shared__ TridTile<R, L, CH> T;
+  const bool xdir = (int)blockIdx.x < nxblk;
+  const int blk = xdir ? blockIdx.x : blockIdx.x - nxblk;
+  const int nlines_tot = xdir ? (g.Je - g.Jb + 1) : (g.Ie - g.Ib + 1);
+  const int n = xdir ? (g.Ie - g.Ib + 1) : (g.Je - g.Jb + 1);
+  const int line0 = blk * L;
+  const int nl = min(L, nlines_tot - line0);
+  const R *A = xdir ? d.xa : d.ya;
+  const R *C = xdir ? d.xc : d.yc;
+  const R *D = xdir ? d.xd : d.yd;
+  R *CP = (R *)C;
+  R *DP = (R *)D;
+  R *Z = xdir ? d.U : d.V;
+  const int t = threadIdx.x;
+  constexpr int PER = (L * CH + 255) / 256;
+  // global index of (line, k)
+  auto gix = [&](int l, int k) -> size_t {
+    if (xdir) return (size_t)(g.Jb + line0 + l - 1) * g.P + (size_t)(g.Ib + k - 1);
+    return (size_t)(g.Jb + k - 1) * g.P + (size_t)(g.Ib + line0 + l - 1);
+  };
+  auto emap = [&](int e, int &l, int &k) {
+    if (xdir) { l = e / CH; k = e % CH; } else { l = e % L; k = e / L; }
+  };
+  const int nch = (n + CH - 1) / CH;
+  R ra[PER], rc[PER], rd[PER];
+  auto load_regs = [&](int ch, const R *X, const R *Y, const R *W) {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      int e = t + q * 256, l, k;
+      emap(e, l, k);
+      int kk = ch * CH + k;
+      if (e < L * CH && l < nl && kk < n) {
+        size_t o = gix(l, kk);
+        ra[q] = X ? X[o] : (R)0;
+        rc[q] = Y[o];
+        rd[q] = W[o];
+      }
+    }
+  };
+  auto regs_to_lds = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      int e = t + q * 256, l, k;
+      emap(e, l, k);
+      if (e < L * CH) { T.a[b][l][k] = ra[q]; T.c[b][l][k] = rc[q]; T.d[b][l][k] = rd[q]; }
+    }
+  };
+  // ---- forward sweep
+  R cprev = (R)0, dprev = (R)0;
+  load_regs(0, A, C, D);
+  regs_to_lds(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ch++) {
+    const int b = ch & 1;
+    if (ch + 1 < nch) load_regs(ch + 1, A, C, D);
+    if (t < nl) {
+      const int kmax = min(CH, n - ch * CH);
+      for (int k = 0; k < kmax; k++) {
+        R a = T.a[b][t][k], c = T.c[b][t][k], dd = T.d[b][t][k];
+        if (ch * CH + k > 0 && a != (R)0) {
+          R den = ddiv((R)1, a) - cprev;
+          c = ddiv(ddiv(c, a), den);
+          dd = ddiv(ddiv(dd, a) - dprev, ddiv((R)1, a) - cprev);
+        }
+        T.c[b][t][k] = c;
+        T.d[b][t][k] = dd;
+        cprev = c;
+        dprev = dd;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; q++) {  // store c', d' of this chunk
+      int e = t + q * 256, l, k;
+      emap(e, l, k);
+      int kk = ch * CH + k;
+      if (e < L * CH && l < nl && kk < n) {
+        size_t o = gix(l, kk);
+        CP[o] = T.c[b][l][k];
+        DP[o] = T.d[b][l][k];
+      }
+    }
+    if (ch + 1 < nch) regs_to_lds(b ^ 1);
+    __syncthreads();
+  }
+  // ---- backward sweep, chunks in reverse
+  R z = (R)0;
+  load_regs(nch - 1, nullptr, CP, DP);
+  regs_to_lds((nch - 1) & 1);
+  __syncthreads();
+  for (int ch = nch - 1; ch >= 0; ch--) {
+    const int b = ch & 1;
+    if (ch - 1 >= 0) load_regs(ch - 1, nullptr, CP, DP);
+    if (t < nl) {
+      const int kmax = min(CH, n - ch * CH);
+      for (int k = kmax - 1; k >= 0; k--) {
+        const int kk = ch * CH + k;
+        R dd = T.d[b][t][k];
+        z = (kk == n - 1) ? dd : dd - T.c[b][t][k] * z;
+        T.a[b][t][k] = z;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      int e = t + q * 256, l, k;
+      emap(e, l, k);
+      int kk = ch * CH + k;
+      if (e < L * CH && l < nl && kk < n) Z[gix(l, kk)] = T.a[b][l][k];
+    }
+    if (ch - 1 >= 0) regs_to_lds(b ^ 1);
+    __syncthreads();
+  }
+}
+
 // GET_Eta_U_V_HU_HV tail (etauv_solver.F:568-593): dry cells, HU/HV, Froude
 // cap; without dispersion Ubar/Vbar := HU/HV on all cells.
 template <class R>

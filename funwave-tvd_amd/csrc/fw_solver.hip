@@ -416,9 +416,15 @@ int do_huv(fw_ctx *c, int istage) {
          (R)c->dt);
   if (c->cfg.DISPERSION) {
     int ny = g.Je - g.Jb + 1, nx = g.Ie - g.Ib + 1;
-    int bs = 64;
-    int nxblk = (ny + bs - 1) / bs, nyblk = (nx + bs - 1) / bs;
-    launch(c, "trid", k_trid<R>, dim3(nxblk + nyblk), dim3(bs), d, g, c->prm<R>(), nxblk);
+    if (c->cfg.PERIODIC) {  // short periodic y lines: Sherman-Morrison per thread
+      int bs = 64;
+      int nxblk = (ny + bs - 1) / bs, nyblk = (nx + bs - 1) / bs;
+      launch(c, "trid", k_trid<R>, dim3(nxblk + nyblk), dim3(bs), d, g, c->prm<R>(), nxblk);
+    } else {
+      constexpr int L = sizeof(R) == 4 ? 16 : 8, CH = 128;
+      int nxblk = (ny + L - 1) / L, nyblk = (nx + L - 1) / L;
+      launch(c, "trid", k_trid_lds<R, L, CH>, dim3(nxblk + nyblk), dim3(256), d, g, nxblk);
+    }
   }
   launch(c, "uv_final", k_uv_final<R>, grid2(g.M, g.N, kB2), kB2, d, g, c->prm<R>());
   if (!c->cfg.DISPERSION) {

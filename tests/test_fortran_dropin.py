@@ -1,0 +1,61 @@
+"""Drop-in test: the reference program's own input driver, initialisation
+and output code, with the per-step routines replaced by the FWGPU_ Fortran
+routines (ISO_C_BINDING over libfwgpu.so), writes the same output files as
+the CPU reference on the same input.txt."""
+import filecmp
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from common import REPO, rel_l2
+
+REF = os.path.join(REPO, "oracle", "_ref")
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not os.path.exists(os.path.join(REF, "fwgpu_drv_sp")),
+                                 reason="drop-in driver not built (make -C oracle dropin)")]
+
+
+def _run(binary, case_dir, nsteps, extra=None):
+    env = dict(os.environ, FWREF_NSTEPS=str(nsteps), FWREF_OUTPUT="1", FWREF_DUMP_EVERY=str(nsteps),
+               FWREF_PREFIX="d", MALLOC_PERTURB_="255")
+    env.update(extra or {})
+    r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {binary} input.txt"], cwd=case_dir,
+                       env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+@pytest.mark.parametrize("mode", ["routines", "step"])
+@pytest.mark.parametrize("prec", ["sp", "dp"])
+def test_dropin_writes_reference_output_files(prec, mode, gpu_available):
+    import cases
+    from fwdump import read_dump
+    print(f"\n[drop-in] prec={prec} mode={mode}", flush=True)
+    with tempfile.TemporaryDirectory() as cpu, tempfile.TemporaryDirectory() as gpu:
+        for d in (cpu, gpu):
+            c = cases.tiny2d()
+            c.params.update({"PLOT_INTV": 1.0, "PLOT_START_TIME": 0.0, "SCREEN_INTV": 1.0e9})
+            c.write(d)
+        n = 40
+        _run(os.path.join(REF, f"fwref_{prec}"), cpu, n)
+        _run(os.path.join(REF, f"fwgpu_drv_{prec}"), gpu, n, {"FWGPU_MODE": mode})
+        out_c = sorted(os.listdir(os.path.join(cpu, "output")))
+        out_g = sorted(os.listdir(os.path.join(gpu, "output")))
+        assert out_c == out_g and any(f.startswith("eta_") for f in out_c), (out_c, out_g)
+        same = [f for f in out_c if filecmp.cmp(os.path.join(cpu, "output", f), os.path.join(gpu, "output", f),
+                                                 shallow=False)]
+        print(f"  {len(out_c)} output files, {len(same)} byte-identical")
+        # every field file must be numerically equal to the printed precision
+        for f in out_c:
+            if f.endswith(".txt") or not any(f.startswith(p) for p in ("eta_", "u_", "v_", "mask_")):
+                continue
+            a = np.loadtxt(os.path.join(cpu, "output", f))
+            b = np.loadtxt(os.path.join(gpu, "output", f))
+            assert a.shape == b.shape
+            assert rel_l2(b, a) < 1e-6, f
+        dc = read_dump(os.path.join(cpu, f"d_step{n:06d}.bin"))
+        dg = read_dump(os.path.join(gpu, f"d_step{n:06d}.bin"))
+        assert np.array_equal(dc["MASK"], dg["MASK"])
+        assert rel_l2(dg["Eta"][3:-3, 3:-3], dc["Eta"][3:-3, 3:-3]) < (1e-5 if prec == "sp" else 1e-10)
