@@ -77,6 +77,16 @@ struct Prm {
   int DISPERSION, PERIODIC, VISC_BRK, SHOW_BRK, DIFF_SPONGE, FRIC_SPONGE;
   int DIRECT_SPONGE, ETA_LIMITER, FIXED_DT, wavemaker, NumFreq;
   int ilo, ihi, jlo, jhi;
+  int full_state;  // write every intermediate array (routine-level parity)
+  // 1/DX, 1/DY and whether they are exact (DX, DY powers of two): then
+  // x/DX == x*(1/DX) bit for bit and the division is replaced
+  R rDX, rDY;
+  int dx_pow2, dy_pow2;
+  // constant factors of the tridiagonal rows, evaluated once in the
+  // reference's operation order (etauv_solver.F:402-411, :520-530)
+  R cxa, cxb, cxc, cxd, cya, cyb, cyc, cyd, b1h;
+  R h5x, h5y;       // 0.5/DX, 0.5/DY (sources.F:441-449)
+  R rDXg, rDYg;     // 1/DXg, 1/DYg (etauv_solver.F:124)
 };
 
 }  // namespace fw

@@ -54,6 +54,18 @@ __device__ __forceinline__ float ddiv(float a, float b) { return __fdiv_rn(a, b)
 __device__ __forceinline__ double ddiv(double a, double b) { return a / b; }
 // SIGN(1.0, b) with IEEE signed zero
 template <class R> __device__ __forceinline__ R sign1(R b) { return signbit(b) ? (R)-1 : (R)1; }
+// x / 2 and x * 0.5 are the same correctly rounded value
+template <class R> __device__ __forceinline__ R half(R x) { return x * (R)0.5; }
+// division by a grid spacing: a multiplication when the spacing is a power
+// of two (bitwise the same), a correctly rounded division otherwise
+template <class R>
+struct Dv {
+  R d, r;
+  int pow2;
+};
+template <class R> __device__ __forceinline__ R dv(R x, const Dv<R> &s) { return s.pow2 ? x * s.r : ddiv(x, s.d); }
+template <class R> __device__ __forceinline__ Dv<R> dvx(const Prm<R> &p) { return Dv<R>{p.DX, p.rDX, p.dx_pow2}; }
+template <class R> __device__ __forceinline__ Dv<R> dvy(const Prm<R> &p) { return Dv<R>{p.DY, p.rDY, p.dy_pow2}; }
 
 // --------------------------------------------------------------------------
 // Ghost-cell fills (PHI_COLL bc.F:631-1272, PHI_COLL_VARIABLE_LENGTH
@@ -238,41 +250,41 @@ __global__ void k_copy3(R *d0, const R *s0, R *d1, const R *s1, R *d2, const R *
 // Ut/Vt on all cells, and zeroed cross derivatives on the wall lines.
 // DU = max(Depth,MinDepthFrc) U and DUt are recomputed at each stencil point.
 template <class R>
-__device__ __forceinline__ R dfx(const Geo &g, const R *a, int i, int j, R dx) {
-  return ddiv(ddiv(a[IX(i + 1, j)] - a[IX(i - 1, j)], dx), (R)2);
+__device__ __forceinline__ R dfx(const Geo &g, const R *a, int i, int j, const Dv<R> &dx) {
+  return half(dv(a[IX(i + 1, j)] - a[IX(i - 1, j)], dx));
 }
 template <class R>
-__device__ __forceinline__ R dfy(const Geo &g, const R *a, int i, int j, R dy) {
-  return ddiv(ddiv(a[IX(i, j + 1)] - a[IX(i, j - 1)], dy), (R)2);
+__device__ __forceinline__ R dfy(const Geo &g, const R *a, int i, int j, const Dv<R> &dy) {
+  return half(dv(a[IX(i, j + 1)] - a[IX(i, j - 1)], dy));
 }
 template <class R>
-__device__ __forceinline__ R dfxx(const Geo &g, const R *a, int i, int j, R dx) {
-  return ddiv(ddiv(a[IX(i + 1, j)] - (R)2 * a[IX(i, j)] + a[IX(i - 1, j)], dx), dx);
+__device__ __forceinline__ R dfxx(const Geo &g, const R *a, int i, int j, const Dv<R> &dx) {
+  return dv(dv(a[IX(i + 1, j)] - (R)2 * a[IX(i, j)] + a[IX(i - 1, j)], dx), dx);
 }
 template <class R>
-__device__ __forceinline__ R dfyy(const Geo &g, const R *a, int i, int j, R dy) {
-  return ddiv(ddiv(a[IX(i, j + 1)] - (R)2 * a[IX(i, j)] + a[IX(i, j - 1)], dy), dy);
+__device__ __forceinline__ R dfyy(const Geo &g, const R *a, int i, int j, const Dv<R> &dy) {
+  return dv(dv(a[IX(i, j + 1)] - (R)2 * a[IX(i, j)] + a[IX(i, j - 1)], dy), dy);
 }
 template <class R>
-__device__ __forceinline__ R dfxy(const Geo &g, const R *a, int i, int j, R dx, R dy) {
-  R t1 = ddiv(ddiv(a[IX(i + 1, j + 1)] - a[IX(i + 1, j - 1)], (R)2), dy);
-  R t2 = ddiv(ddiv(a[IX(i - 1, j + 1)] - a[IX(i - 1, j - 1)], (R)2), dy);
-  return ddiv(ddiv(t1 - t2, (R)2), dx);
+__device__ __forceinline__ R dfxy(const Geo &g, const R *a, int i, int j, const Dv<R> &dx, const Dv<R> &dy) {
+  R t1 = dv(half(a[IX(i + 1, j + 1)] - a[IX(i + 1, j - 1)]), dy);
+  R t2 = dv(half(a[IX(i - 1, j + 1)] - a[IX(i - 1, j - 1)]), dy);
+  return dv(half(t1 - t2), dx);
 }
 // stencil of a derived field f(i,j) evaluated on the fly
 template <class R, class Fn>
-__device__ __forceinline__ R gfx(Fn f, int i, int j, R dx) { return ddiv(ddiv(f(i + 1, j) - f(i - 1, j), dx), (R)2); }
+__device__ __forceinline__ R gfx(Fn f, int i, int j, const Dv<R> &dx) { return half(dv(f(i + 1, j) - f(i - 1, j), dx)); }
 template <class R, class Fn>
-__device__ __forceinline__ R gfy(Fn f, int i, int j, R dy) { return ddiv(ddiv(f(i, j + 1) - f(i, j - 1), dy), (R)2); }
+__device__ __forceinline__ R gfy(Fn f, int i, int j, const Dv<R> &dy) { return half(dv(f(i, j + 1) - f(i, j - 1), dy)); }
 template <class R, class Fn>
-__device__ __forceinline__ R gfxx(Fn f, int i, int j, R dx) { return ddiv(ddiv(f(i + 1, j) - (R)2 * f(i, j) + f(i - 1, j), dx), dx); }
+__device__ __forceinline__ R gfxx(Fn f, int i, int j, const Dv<R> &dx) { return dv(dv(f(i + 1, j) - (R)2 * f(i, j) + f(i - 1, j), dx), dx); }
 template <class R, class Fn>
-__device__ __forceinline__ R gfyy(Fn f, int i, int j, R dy) { return ddiv(ddiv(f(i, j + 1) - (R)2 * f(i, j) + f(i, j - 1), dy), dy); }
+__device__ __forceinline__ R gfyy(Fn f, int i, int j, const Dv<R> &dy) { return dv(dv(f(i, j + 1) - (R)2 * f(i, j) + f(i, j - 1), dy), dy); }
 template <class R, class Fn>
-__device__ __forceinline__ R gfxy(Fn f, int i, int j, R dx, R dy) {
-  R t1 = ddiv(ddiv(f(i + 1, j + 1) - f(i + 1, j - 1), (R)2), dy);
-  R t2 = ddiv(ddiv(f(i - 1, j + 1) - f(i - 1, j - 1), (R)2), dy);
-  return ddiv(ddiv(t1 - t2, (R)2), dx);
+__device__ __forceinline__ R gfxy(Fn f, int i, int j, const Dv<R> &dx, const Dv<R> &dy) {
+  R t1 = dv(half(f(i + 1, j + 1) - f(i + 1, j - 1)), dy);
+  R t2 = dv(half(f(i - 1, j + 1) - f(i - 1, j - 1)), dy);
+  return dv(half(t1 - t2), dx);
 }
 
 template <class R>
@@ -281,11 +293,12 @@ __global__ void k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
   int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
-  const R dx = p.DX, dy = p.DY, mdf = p.MDF;
+  const Dv<R> dx = dvx(p), dy = dvy(p);
+  const R mdf = p.MDF;
   const bool g2 = p.G2 > (R)0;
   // ETAT from the previous stage's P, Q (dispersion.F:103-119)
   if (i <= g.M - 1 && j <= g.N - 1) {
-    d.ETAT[o] = -ddiv(d.P[IX(i + 1, j)] - d.P[IX(i, j)], dx) - ddiv(d.Q[IX(i, j + 1)] - d.Q[IX(i, j)], dy);
+    d.ETAT[o] = -dv(d.P[IX(i + 1, j)] - d.P[IX(i, j)], dx) - dv(d.Q[IX(i, j + 1)] - d.Q[IX(i, j)], dy);
   }
   if (g2) {  // :133-142
     d.Ut[o] = ddiv(d.U[o] - d.U0[o], DT);
@@ -346,7 +359,8 @@ __global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p) {
   int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
-  const R B1 = p.B1, B2 = p.B2, b2 = p.b2, g2 = p.G2, dx = p.DX, dy = p.DY;
+  const R B1 = p.B1, B2 = p.B2, b2 = p.b2, g2 = p.G2;
+  const Dv<R> dx = dvx(p), dy = dvy(p);
   const R c13 = (R)1 / (R)3, c12 = (R)1 / (R)2, c16 = (R)1 / (R)6;
   {
     R dep = d.Depth[o];
@@ -373,19 +387,19 @@ __global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p) {
 #define A_(n, a, b) d.n[IX(a, b)]
   R UxxVxy = A_(Uxx, i, j) + A_(Vxy, i, j);
   R UxyVyy = A_(Uxy, i, j) + A_(Vyy, i, j);
-  R UxxVxy_x = ddiv(ddiv(A_(Uxx, i + 1, j) + A_(Vxy, i + 1, j) - A_(Uxx, i - 1, j) - A_(Vxy, i - 1, j), (R)2), dx);
-  R UxxVxy_y = ddiv(ddiv(A_(Uxx, i, j + 1) + A_(Vxy, i, j + 1) - A_(Uxx, i, j - 1) - A_(Vxy, i, j - 1), (R)2), dy);
-  R UxyVyy_x = ddiv(ddiv(A_(Uxy, i + 1, j) + A_(Vyy, i + 1, j) - A_(Uxy, i - 1, j) - A_(Vyy, i - 1, j), (R)2), dx);
-  R UxyVyy_y = ddiv(ddiv(A_(Uxy, i, j + 1) + A_(Vyy, i, j + 1) - A_(Uxy, i, j - 1) - A_(Vyy, i, j - 1), (R)2), dy);
+  R UxxVxy_x = dv(half(A_(Uxx, i + 1, j) + A_(Vxy, i + 1, j) - A_(Uxx, i - 1, j) - A_(Vxy, i - 1, j)), dx);
+  R UxxVxy_y = dv(half(A_(Uxx, i, j + 1) + A_(Vxy, i, j + 1) - A_(Uxx, i, j - 1) - A_(Vxy, i, j - 1)), dy);
+  R UxyVyy_x = dv(half(A_(Uxy, i + 1, j) + A_(Vyy, i + 1, j) - A_(Uxy, i - 1, j) - A_(Vyy, i - 1, j)), dx);
+  R UxyVyy_y = dv(half(A_(Uxy, i, j + 1) + A_(Vyy, i, j + 1) - A_(Uxy, i, j - 1) - A_(Vyy, i, j - 1)), dy);
   R HUxxHVxy = A_(DUxx, i, j) + A_(DVxy, i, j);
   R HUxyHVyy = A_(DUxy, i, j) + A_(DVyy, i, j);
-  R HUxxHVxy_x = ddiv(ddiv(A_(DUxx, i + 1, j) + A_(DVxy, i + 1, j) - A_(DUxx, i - 1, j) - A_(DVxy, i - 1, j), (R)2), dx);
-  R HUxxHVxy_y = ddiv(ddiv(A_(DUxx, i, j + 1) + A_(DVxy, i, j + 1) - A_(DUxx, i, j - 1) - A_(DVxy, i, j - 1), (R)2), dy);
-  R HUxyHVyy_x = ddiv(ddiv(A_(DUxy, i + 1, j) + A_(DVyy, i + 1, j) - A_(DUxy, i - 1, j) - A_(DVyy, i - 1, j), (R)2), dx);
-  R HUxyHVyy_y = ddiv(ddiv(A_(DUxy, i, j + 1) + A_(DVyy, i, j + 1) - A_(DUxy, i, j - 1) - A_(DVyy, i, j - 1), (R)2), dy);
+  R HUxxHVxy_x = dv(half(A_(DUxx, i + 1, j) + A_(DVxy, i + 1, j) - A_(DUxx, i - 1, j) - A_(DVxy, i - 1, j)), dx);
+  R HUxxHVxy_y = dv(half(A_(DUxx, i, j + 1) + A_(DVxy, i, j + 1) - A_(DUxx, i, j - 1) - A_(DVxy, i, j - 1)), dy);
+  R HUxyHVyy_x = dv(half(A_(DUxy, i + 1, j) + A_(DVyy, i + 1, j) - A_(DUxy, i - 1, j) - A_(DVyy, i - 1, j)), dx);
+  R HUxyHVyy_y = dv(half(A_(DUxy, i, j + 1) + A_(DVyy, i, j + 1) - A_(DUxy, i, j - 1) - A_(DVyy, i, j - 1)), dy);
   R rh = A_(Depth, i, j);
-  R rhx = ddiv(ddiv(A_(Depth, i + 1, j) - A_(Depth, i - 1, j), (R)2), dx);
-  R rhy = ddiv(ddiv(A_(Depth, i, j + 1) - A_(Depth, i, j - 1), (R)2), dy);
+  R rhx = dv(half(A_(Depth, i + 1, j) - A_(Depth, i - 1, j)), dx);
+  R rhy = dv(half(A_(Depth, i, j + 1) - A_(Depth, i, j - 1)), dy);
   R reta = A_(Eta, i, j);
   R ex = A_(ETAx, i, j), ey = A_(ETAy, i, j), et = A_(ETAT, i, j);
   R Utx_ = A_(Utx, i, j), Vty_ = A_(Vty, i, j);
@@ -546,13 +560,15 @@ __global__ void k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
     muscl(d.HU, d.MASK, o, 1, huL, huR);
     muscl(d.HV, d.MASK, o, 1, hvL, hvR);
     muscl(d.Eta, d.MASK, o, 1, eL, eR);
-    d.UxL[o] = uL; d.UxR[o] = uR; d.VxL[o] = vL; d.VxR[o] = vR;
-    d.HUxL[o] = huL; d.HUxR[o] = huR; d.HVxL[o] = hvL; d.HVxR[o] = hvR;
-    d.EtaRxL[o] = eL; d.EtaRxR[o] = eR;
+    if (p.full_state) {
+      d.UxL[o] = uL; d.UxR[o] = uR; d.VxL[o] = vL; d.VxR[o] = vR;
+      d.HUxL[o] = huL; d.HUxR[o] = huR; d.HVxL[o] = hvL; d.HVxR[o] = hvR;
+      d.EtaRxL[o] = eL; d.EtaRxR[o] = eR;
+    }
     if (p.DISPERSION) {
       muscl(d.U4, d.MASK, o, 1, u4L, u4R);
       muscl(d.V4, d.MASK, o, 1, v4L, v4R);
-      d.U4xL[o] = u4L; d.U4xR[o] = u4R; d.V4xL[o] = v4L; d.V4xR[o] = v4R;
+      if (p.full_state) { d.U4xL[o] = u4L; d.U4xR[o] = u4R; d.V4xL[o] = v4L; d.V4xR[o] = v4R; }
     } else {
       u4L = d.U4xL[o]; u4R = d.U4xR[o]; v4L = d.V4xL[o]; v4R = d.V4xR[o];
     }
@@ -564,7 +580,7 @@ __global__ void k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
   }
   const R dpx = d.DepthX[o];
   R hL = eL + dpx, hR = eR + dpx;
-  d.HxL[o] = hL; d.HxR[o] = hR;
+  if (p.full_state || !comp) { d.HxL[o] = hL; d.HxR[o] = hR; }
   const R m9u = (R)d.MASK9[IX(i <= g.M ? i : g.M, j)];
   R pl = huL + g1 * m9u * hL * u4L;
   R pr = huR + g1 * m9u * hR * u4R;
@@ -572,7 +588,7 @@ __global__ void k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
   R fxr = g3 * pr * (uR + g1 * m9u * u4R) + (R)0.5 * GRAV * (eR * eR * g3 + (R)2 * eR * dpx);
   R gxl = hL * (uL + g1 * m9u * u4L) * (vL + g1 * m9u * v4L) * g3;
   R gxr = hR * (uR + g1 * m9u * u4R) * (vR + g1 * m9u * v4R) * g3;
-  d.PL[o] = pl; d.PR[o] = pr; d.FxL[o] = fxl; d.FxR[o] = fxr; d.GxL[o] = gxl; d.GxR[o] = gxr;
+  if (p.full_state || !comp) { d.PL[o] = pl; d.PR[o] = pr; d.FxL[o] = fxl; d.FxR[o] = fxr; d.GxL[o] = gxl; d.GxR[o] = gxr; }
   // wave speed of the nearest computed face (ghost faces copy it)
   int ci = i < 4 ? 4 : (i > M1 - 3 ? M1 - 3 : i);
   int cj = j < 4 ? 4 : (j > g.N - 3 ? g.N - 3 : j);
@@ -586,7 +602,7 @@ __global__ void k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
   } else {
     xface_speed(d, g, p, ci, cj, sl, sr);
   }
-  d.SxL[o] = sl; d.SxR[o] = sr;
+  if (p.full_state) { d.SxL[o] = sl; d.SxR[o] = sr; }
   R P = hll(sl, sr, pl, pr, eL, eR);
   R Fx = hll(sl, sr, fxl, fxr, huL, huR);
   R Gx = hll(sl, sr, gxl, gxr, hvL, hvR);
@@ -629,13 +645,15 @@ __global__ void k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
     muscl(d.HV, d.MASK, o, s, hvL, hvR);
     muscl(d.HU, d.MASK, o, s, huL, huR);
     muscl(d.Eta, d.MASK, o, s, eL, eR);
-    d.UyL[o] = uL; d.UyR[o] = uR; d.VyL[o] = vL; d.VyR[o] = vR;
-    d.HUyL[o] = huL; d.HUyR[o] = huR; d.HVyL[o] = hvL; d.HVyR[o] = hvR;
-    d.EtaRyL[o] = eL; d.EtaRyR[o] = eR;
+    if (p.full_state) {
+      d.UyL[o] = uL; d.UyR[o] = uR; d.VyL[o] = vL; d.VyR[o] = vR;
+      d.HUyL[o] = huL; d.HUyR[o] = huR; d.HVyL[o] = hvL; d.HVyR[o] = hvR;
+      d.EtaRyL[o] = eL; d.EtaRyR[o] = eR;
+    }
     if (p.DISPERSION) {
       muscl(d.V4, d.MASK, o, s, v4L, v4R);
       muscl(d.U4, d.MASK, o, s, u4L, u4R);
-      d.U4yL[o] = u4L; d.U4yR[o] = u4R; d.V4yL[o] = v4L; d.V4yR[o] = v4R;
+      if (p.full_state) { d.U4yL[o] = u4L; d.U4yR[o] = u4R; d.V4yL[o] = v4L; d.V4yR[o] = v4R; }
     } else {
       u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
     }
@@ -647,7 +665,7 @@ __global__ void k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
   }
   const R dpy = d.DepthY[o];
   R hL = eL + dpy, hR = eR + dpy;
-  d.HyL[o] = hL; d.HyR[o] = hR;
+  if (p.full_state || !comp) { d.HyL[o] = hL; d.HyR[o] = hR; }
   const R m9v = (R)d.MASK9[IX(i, j <= g.N ? j : g.N)];
   R ql = hvL + g1 * m9v * hL * v4L;
   R qr = hvR + g1 * m9v * hR * v4R;
@@ -655,7 +673,7 @@ __global__ void k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
   R gyr = g3 * qr * (vR + g1 * m9v * v4R) + (R)0.5 * GRAV * (eR * eR * g3 + (R)2 * eR * dpy);
   R fyl = hL * (uL + g1 * m9v * u4L) * (vL + g1 * m9v * v4L) * g3;
   R fyr = hR * (uR + g1 * m9v * u4R) * (vR + g1 * m9v * v4R) * g3;
-  d.QL[o] = ql; d.QR[o] = qr; d.GyL[o] = gyl; d.GyR[o] = gyr; d.FyL[o] = fyl; d.FyR[o] = fyr;
+  if (p.full_state || !comp) { d.QL[o] = ql; d.QR[o] = qr; d.GyL[o] = gyl; d.GyR[o] = gyr; d.FyL[o] = fyl; d.FyR[o] = fyr; }
   int ci = i < 4 ? 4 : (i > g.M - 3 ? g.M - 3 : i);
   int cj = j < 4 ? 4 : (j > N1 - 3 ? N1 - 3 : j);
   R sl, sr;
@@ -668,7 +686,7 @@ __global__ void k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
   } else {
     yface_speed(d, g, p, ci, cj, sl, sr);
   }
-  d.SyL[o] = sl; d.SyR[o] = sr;
+  if (p.full_state) { d.SyL[o] = sl; d.SyR[o] = sr; }
   R Q = hll(sl, sr, ql, qr, eL, eR);
   R Fy = hll(sl, sr, fyl, fyr, huL, huR);
   R Gy = hll(sl, sr, gyl, gyr, hvL, hvR);
@@ -703,7 +721,8 @@ __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
   int j = g.Jb + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.Ie || j > g.Je) return;
   size_t o = IX(i, j);
-  const R dx = p.DX, dy = p.DY, mdf = p.MDF, g1 = p.G1, g2 = p.G2;
+  const Dv<R> dx = dvx(p), dy = dvy(p);
+  const R mdf = p.MDF, g1 = p.G1, g2 = p.G2;
   R wm;
   if (p.wavemaker != 0 && i >= p.ilo && i <= p.ihi && j >= p.jlo && j <= p.jhi) {
     int bi = i - p.ilo, bj = j - p.jlo;
@@ -727,24 +746,24 @@ __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
   d.FrcInsX[o] = fix;
   R m9r = (R)d.MASK9[o];
   R hmax = rmax(d.H[o], mdf);
-  R divpq = ddiv(d.P[IX(i + 1, j)] - d.P[o], dx) + ddiv(d.Q[IX(i, j + 1)] - d.Q[o], dy);
+  R divpq = dv(d.P[IX(i + 1, j)] - d.P[o], dx) + dv(d.Q[IX(i, j + 1)] - d.Q[o], dy);
   const R *U4 = d.U4, *V4 = d.V4, *U = d.U, *V = d.V;
   R mk = (R)d.MASK[o];
-  R sx = ddiv(GRAV * (d.Eta[o]), dx) * (d.DepthX[IX(i + 1, j)] - d.DepthX[o]) * mk + fix
-         + g1 * m9r * hmax * (ddiv(u * (R)0.5 * (U4[IX(i + 1, j)] - U4[IX(i - 1, j)]), dx)
-                              + ddiv(v * (R)0.5 * (U4[IX(i, j + 1)] - U4[IX(i, j - 1)]), dy)
-                              + ddiv(U4[o] * (R)0.5 * (U[IX(i + 1, j)] - U[IX(i - 1, j)]), dx)
-                              + ddiv(V4[o] * (R)0.5 * (U[IX(i, j + 1)] - U[IX(i, j - 1)]), dy)
+  R sx = dv(GRAV * (d.Eta[o]), dx) * (d.DepthX[IX(i + 1, j)] - d.DepthX[o]) * mk + fix
+         + g1 * m9r * hmax * (dv(u * (R)0.5 * (U4[IX(i + 1, j)] - U4[IX(i - 1, j)]), dx)
+                              + dv(v * (R)0.5 * (U4[IX(i, j + 1)] - U4[IX(i, j - 1)]), dy)
+                              + dv(U4[o] * (R)0.5 * (U[IX(i + 1, j)] - U[IX(i - 1, j)]), dx)
+                              + dv(V4[o] * (R)0.5 * (U[IX(i, j + 1)] - U[IX(i, j - 1)]), dy)
                               - g2 * m9r * (d.U1pp[o] + d.U2[o] + d.U3[o]))
          + g1 * m9r * divpq * (U4[o] - d.U1p[o]) + wm * u;
   if (p.FRIC_SPONGE) sx = sx - d.CD_4_SPONGE[o] * u * spd * d.Depth[o];
   R fiy = -(d.Cd[o] * v * spd);
   d.FrcInsY[o] = fiy;
-  R sy = ddiv(GRAV * (d.Eta[o]), dy) * (d.DepthY[IX(i, j + 1)] - d.DepthY[o]) * mk + fiy
-         + g1 * m9r * hmax * (ddiv(u * (R)0.5 * (V4[IX(i + 1, j)] - V4[IX(i - 1, j)]), dx)
-                              + ddiv(v * (R)0.5 * (V4[IX(i, j + 1)] - V4[IX(i, j - 1)]), dy)
-                              + ddiv(U4[o] * (R)0.5 * (V[IX(i + 1, j)] - V[IX(i - 1, j)]), dx)
-                              + ddiv(V4[o] * (R)0.5 * (V[IX(i, j + 1)] - V[IX(i, j - 1)]), dy)
+  R sy = dv(GRAV * (d.Eta[o]), dy) * (d.DepthY[IX(i, j + 1)] - d.DepthY[o]) * mk + fiy
+         + g1 * m9r * hmax * (dv(u * (R)0.5 * (V4[IX(i + 1, j)] - V4[IX(i - 1, j)]), dx)
+                              + dv(v * (R)0.5 * (V4[IX(i, j + 1)] - V4[IX(i, j - 1)]), dy)
+                              + dv(U4[o] * (R)0.5 * (V[IX(i + 1, j)] - V[IX(i - 1, j)]), dx)
+                              + dv(V4[o] * (R)0.5 * (V[IX(i, j + 1)] - V[IX(i, j - 1)]), dy)
                               - g2 * m9r * (d.V1pp[o] + d.V2[o] + d.V3[o]))
          + g1 * m9r * divpq * (V4[o] - d.V1p[o]) + wm * v;
   if (p.FRIC_SPONGE) sy = sy - d.CD_4_SPONGE[o] * v * spd * d.Depth[o];
@@ -759,13 +778,13 @@ __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
     R sxp = (R)(nuv(i + 1, j) + n0), sxm = (R)(nuv(i - 1, j) + n0);
     R syp = (R)(nuv(i, j + 1) + n0), sym = (R)(nuv(i, j - 1) + n0);
     const R *HU = d.HU, *HV = d.HV;
-    R h5x = ddiv((R)0.5, dx), h5y = ddiv((R)0.5, dy);
-    R bx = h5x * (ddiv(sxp * (R)1, dx) * (HU[IX(i + 1, j)] - HU[o]) - ddiv(sxm * (R)1, dx) * (HU[o] - HU[IX(i - 1, j)]))
-           + h5y * (ddiv(syp * (R)1, dy) * (HU[IX(i, j + 1)] - HU[o]) - ddiv(sym * (R)1, dy) * (HU[o] - HU[IX(i, j - 1)]));
+    const R h5x = p.h5x, h5y = p.h5y;
+    R bx = h5x * (dv(sxp * (R)1, dx) * (HU[IX(i + 1, j)] - HU[o]) - dv(sxm * (R)1, dx) * (HU[o] - HU[IX(i - 1, j)]))
+           + h5y * (dv(syp * (R)1, dy) * (HU[IX(i, j + 1)] - HU[o]) - dv(sym * (R)1, dy) * (HU[o] - HU[IX(i, j - 1)]));
     d.BreakSourceX[o] = bx;
     sx = sx + bx;
-    R by = h5x * (ddiv(sxp * (R)1, dx) * (HV[IX(i + 1, j)] - HV[o]) - ddiv(sxm * (R)1, dx) * (HV[o] - HV[IX(i - 1, j)]))
-           + h5y * (ddiv(syp * (R)1, dy) * (HV[IX(i, j + 1)] - HV[o]) - ddiv(sym * (R)1, dy) * (HV[o] - HV[IX(i, j - 1)]));
+    R by = h5x * (dv(sxp * (R)1, dx) * (HV[IX(i + 1, j)] - HV[o]) - dv(sxm * (R)1, dx) * (HV[o] - HV[IX(i - 1, j)]))
+           + h5y * (dv(syp * (R)1, dy) * (HV[IX(i, j + 1)] - HV[o]) - dv(sym * (R)1, dy) * (HV[o] - HV[IX(i, j - 1)]));
     d.BreakSourceY[o] = by;
     sy = sy + by;
   }
@@ -797,10 +816,10 @@ __global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT) {
     d.V_davg[o] = ddiv(qc, rmax(hcur, mdf));
     const R nl = (R)-1, nr = (R)1;
     R fl = P[o], fr = P[IX(i + 1, j)], fb = Q[o], ft = Q[IX(i, j + 1)];
-    R r1 = -(ddiv((R)1, p.DXg) * (fr * nr + fl * nl)) - ddiv((R)1, p.DYg) * (ft * nr + fb * nl);
+    R r1 = -(p.rDXg * (fr * nr + fl * nl)) - p.rDYg * (ft * nr + fb * nl);
     if (p.wavemaker != 0) {
       R xmk = (R)(i - g.Ib) * p.DXg, ymk = (R)(j - g.Jb) * p.DYg;
-      if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < ddiv(p.YW, (R)2)) r1 = r1 + d.WaveMaker_Mass[o];
+      if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < half(p.YW)) r1 = r1 + d.WaveMaker_Mass[o];
     }
     d.Wsurf[o] = r1;
     eta = al * d.Eta0[o] + be * (eta + DT * r1);
@@ -810,26 +829,26 @@ __global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT) {
     }
     d.Eta[o] = eta;
     fl = d.Fx[o]; fr = d.Fx[IX(i + 1, j)]; fb = d.Fy[o]; ft = d.Fy[IX(i, j + 1)];
-    R r2 = -(ddiv((R)1, p.DX) * (fr * nr + fl * nl)) - ddiv((R)1, p.DY) * (ft * nr + fb * nl) + d.SourceX[o];
+    R r2 = -(p.rDX * (fr * nr + fl * nl)) - p.rDY * (ft * nr + fb * nl) + d.SourceX[o];
     R ub = al * d.Ubar0[o] + be * (d.Ubar[o] + DT * r2);
     d.Ubar[o] = ub;
     fl = d.Gx[o]; fr = d.Gx[IX(i + 1, j)]; fb = d.Gy[o]; ft = d.Gy[IX(i, j + 1)];
-    R r3 = -(ddiv((R)1, p.DX) * (fr * nr + fl * nl)) - ddiv((R)1, p.DY) * (ft * nr + fb * nl) + d.SourceY[o];
+    R r3 = -(p.rDX * (fr * nr + fl * nl)) - p.rDY * (ft * nr + fb * nl) + d.SourceY[o];
     R vb = al * d.Vbar0[o] + be * (d.Vbar[o] + DT * r3);
     d.Vbar[o] = vb;
     R h = eta * p.G3 + d.Depth[o];
     d.H[o] = h;
     if (p.DISPERSION) {
-      const R dx = p.DX, dy = p.DY, b1 = p.b1, b2 = p.b2;
+      const R b2 = p.b2;
       const int mk = d.MASK[o];
       const R gm = p.G1 * (R)d.MASK9[o];
       R dep = rmax(d.Depth[o], mdf);
       {
         R depl = rmax(d.Depth[IX(i - 1, j)], mdf), depr = rmax(d.Depth[IX(i + 1, j)], mdf);
-        R t1 = gm * (ddiv(ddiv(ddiv(b1, (R)2), dx), dx) * dep * dep + ddiv(ddiv(b2, dx), dx) * depl * dep);
-        R t2 = (R)1 + gm * (-(ddiv(ddiv(b1, dx), dx) * dep * dep) - ddiv(ddiv((R)2 * b2, dx), dx) * dep * dep);
-        R t3 = gm * (ddiv(ddiv(ddiv(b1, (R)2), dx), dx) * dep * dep + ddiv(ddiv(b2, dx), dx) * dep * depr);
-        R t4 = ddiv(ub * (R)mk, rmax(h, mdf)) + gm * (-(ddiv(b1, (R)2) * dep * dep * d.Vxy[o]) - b2 * dep * d.DVxy[o]);
+        R t1 = gm * (p.cxa * dep * dep + p.cxb * depl * dep);
+        R t2 = (R)1 + gm * (-(p.cxc * dep * dep) - p.cxd * dep * dep);
+        R t3 = gm * (p.cxa * dep * dep + p.cxb * dep * depr);
+        R t4 = ddiv(ub * (R)mk, rmax(h, mdf)) + gm * (-(p.b1h * dep * dep * d.Vxy[o]) - b2 * dep * d.DVxy[o]);
         if (t2 != (R)0 || mk > 0) {
           d.xa[o] = ddiv(t1, t2); d.xc[o] = ddiv(t3, t2); d.xd[o] = ddiv(t4, t2);
         } else {
@@ -838,10 +857,10 @@ __global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT) {
       }
       {
         R depl = rmax(d.Depth[IX(i, j - 1)], mdf), depr = rmax(d.Depth[IX(i, j + 1)], mdf);
-        R t1 = gm * (ddiv(ddiv(ddiv(b1, (R)2), dy), dy) * dep * dep + ddiv(ddiv(b2, dy), dy) * depl * dep);
-        R t2 = (R)1 + gm * (-(ddiv(ddiv(b1, dy), dy) * dep * dep) - ddiv(ddiv((R)2 * b2, dy), dy) * dep * dep);
-        R t3 = gm * (ddiv(ddiv(ddiv(b1, (R)2), dy), dy) * dep * dep + ddiv(ddiv(b2, dy), dy) * dep * depr);
-        R t4 = ddiv(vb * (R)mk, rmax(h, mdf)) + gm * (-(ddiv(b1, (R)2) * dep * dep * d.Uxy[o]) - b2 * dep * d.DUxy[o]);
+        R t1 = gm * (p.cya * dep * dep + p.cyb * depl * dep);
+        R t2 = (R)1 + gm * (-(p.cyc * dep * dep) - p.cyd * dep * dep);
+        R t3 = gm * (p.cya * dep * dep + p.cyb * dep * depr);
+        R t4 = ddiv(vb * (R)mk, rmax(h, mdf)) + gm * (-(p.b1h * dep * dep * d.Uxy[o]) - b2 * dep * d.DUxy[o]);
         if (t2 != (R)0 || mk > 0) {
           d.ya[o] = ddiv(t1, t2); d.yc[o] = ddiv(t3, t2); d.yd[o] = ddiv(t4, t2);
         } else {
@@ -1216,7 +1235,7 @@ template <class R>
 __device__ __forceinline__ R brk_nu(const Dev<R> &d, const Geo &g, const Prm<R> &p, const BrkIn<R> &b, size_t o, int i,
                                     int j, R age) {
   R xmk = (R)(i - g.Ib) * p.DX, ymk = (R)(j - g.Jb) * p.DY;
-  if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < ddiv(p.YW, (R)2)) {
+  if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < half(p.YW)) {
     if (b.et > rmin(b.t2, p.WKC * b.t3)) {
       R cap1 = (R)1 * (rmax(d.Depth[o], p.MDF) + d.Eta[o]);
       return cap1 * p.WKC * b.t3 + p.NUB;

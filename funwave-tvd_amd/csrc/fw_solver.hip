@@ -154,6 +154,30 @@ void fill_prm(fw_ctx *c) {
   p.DIRECT_SPONGE = f.DIRECT_SPONGE; p.ETA_LIMITER = f.ETA_LIMITER; p.FIXED_DT = f.FIXED_DT;
   p.wavemaker = f.wavemaker; p.NumFreq = f.NumFreq;
   p.ilo = f.ilo; p.ihi = f.ihi; p.jlo = f.jlo; p.jhi = f.jhi;
+  // exact reciprocals and constant row factors, in the working precision and
+  // the reference's operation order
+  auto pow2 = [](R x) {
+    int e;
+    R m = std::frexp(x, &e);
+    return x > (R)0 && m == (R)0.5 && std::isnormal((R)1 / x);
+  };
+  p.rDX = (R)1 / p.DX;
+  p.rDY = (R)1 / p.DY;
+  p.dx_pow2 = pow2(p.DX);
+  p.dy_pow2 = pow2(p.DY);
+  p.rDXg = (R)1 / p.DXg;
+  p.rDYg = (R)1 / p.DYg;
+  p.h5x = (R)0.5 / p.DX;
+  p.h5y = (R)0.5 / p.DY;
+  p.b1h = p.b1 / (R)2;
+  p.cxa = p.b1 / (R)2 / p.DX / p.DX;
+  p.cxb = p.b2 / p.DX / p.DX;
+  p.cxc = p.b1 / p.DX / p.DX;
+  p.cxd = (R)2 * p.b2 / p.DX / p.DX;
+  p.cya = p.b1 / (R)2 / p.DY / p.DY;
+  p.cyb = p.b2 / p.DY / p.DY;
+  p.cyc = p.b1 / p.DY / p.DY;
+  p.cyd = (R)2 * p.b2 / p.DY / p.DY;
 }
 
 size_t plane_elems(const Geo &g) { return (size_t)g.P * (size_t)(g.N + 1); }
@@ -789,6 +813,16 @@ int fw_step(fw_ctx *c, int nsteps, double *dt, double *time) {
 int fw_reduce_stats(fw_ctx *c, fw_stats *out) {
   if (!c || !out) return FW_E_INVALID;
   return DISPATCH(c, do_stats)(c, out);
+}
+
+int fw_set_option(fw_ctx *c, int option, int value) {
+  if (!c) return FW_E_INVALID;
+  if (option == FW_OPT_FULL_STATE) {
+    c->pf.full_state = value != 0;
+    c->pd.full_state = value != 0;
+    return FW_OK;
+  }
+  return fail(c, FW_E_INVALID, "unknown option");
 }
 
 int fw_profile_enable(fw_ctx *c, int on) {

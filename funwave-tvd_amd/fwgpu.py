@@ -27,6 +27,7 @@ DBL_CFG = ["DX", "DY", "DXg", "DYg", "CFL", "MinDepth", "MinDepthFrc", "Gamma1",
            "CrestLimit", "TroughLimit", "D_gen", "Beta_gen", "rlamda", "Tperiod",
            "Time_ramp", "FreqPeak"]
 
+FW_OPT_FULL_STATE = 1
 FW_OK, FW_E_INVALID, FW_E_HIP, FW_E_NOMEM, FW_E_BLOWUP, FW_E_NOTFOUND, FW_E_BREAK_ITER = 0, -1, -2, -3, -4, -5, -6
 WK = {"WK_REG": 1, "WK_DATA2D": 2}
 NO_SOURCE_WAVEMAKERS = ("INI_SOL", "INI_REC", "INI_GAU", "INI_OTH", "N_WAVE", "NONE")
@@ -38,7 +39,7 @@ EXPORTS = ["fw_create", "fw_destroy", "fw_strerror", "fw_synchronize",
            "fw_save_stage0", "fw_cal_dispersion", "fw_fluxes", "fw_source_terms",
            "fw_estimate_huv", "fw_update_mask", "fw_wave_breaking",
            "fw_sponge_damping", "fw_check_blowup", "fw_step", "fw_reduce_stats",
-           "fw_profile_enable", "fw_profile_reset", "fw_profile_get",
+           "fw_set_option", "fw_profile_enable", "fw_profile_reset", "fw_profile_get",
            "fw_profile_names", "fw_time_steps"]
 
 
@@ -93,6 +94,7 @@ def load_library(path: str = LIB_PATH):
     lib.fw_step.argtypes = [vp, ip, dp, dp]
     lib.fw_reduce_stats.argtypes = [vp, ct.POINTER(Stats)]
     lib.fw_profile_enable.argtypes = [vp, ip]
+    lib.fw_set_option.argtypes = [vp, ip, ip]
     lib.fw_profile_get.argtypes = [vp, ct.c_char_p, ct.POINTER(ct.c_int64), dp]
     lib.fw_profile_names.argtypes = [vp, ct.c_char_p, ip]
     lib.fw_time_steps.argtypes = [vp, ip, dp]
@@ -129,7 +131,7 @@ def config_from_state(st: dict, real_bytes: int) -> Config:
 class Solver:
     """One HBM-resident FUNWAVE-TVD domain on the current GPU."""
 
-    def __init__(self, state: dict, precision: str = "sp", upload: bool = True):
+    def __init__(self, state: dict, precision: str = "sp", upload: bool = True, full_state: bool = False):
         self.lib = load_library()
         self.rb = 8 if precision == "dp" else 4
         self.rdt = np.float64 if self.rb == 8 else np.float32
@@ -139,6 +141,8 @@ class Solver:
         if rc != FW_OK:
             raise FwError(rc, "fw_create failed (configuration rejected or device error)")
         self.M, self.N = self.cfg.Mloc, self.cfg.Nloc
+        if full_state:
+            self._check(self.lib.fw_set_option(self.ctx, FW_OPT_FULL_STATE, 1))
         if upload:
             self.upload_state(state)
 

@@ -114,6 +114,14 @@ int fw_step(fw_ctx *ctx, int nsteps, double *dt, double *time);
 /* FP64 mass / energy / extrema budget of the interior. */
 int fw_reduce_stats(fw_ctx *ctx, fw_stats *out);
 
+/* --- options ------------------------------------------------------------ */
+/* FW_OPT_FULL_STATE = 1 keeps every intermediate array of MODULE GLOBAL up to
+ * date (face states UxL..GyR, HxL.., SxL.., the wave-speed copies), for
+ * routine-by-routine comparison with the reference.  Default 0: only the
+ * arrays a later routine reads are written; results are identical. */
+enum { FW_OPT_FULL_STATE = 1 };
+int fw_set_option(fw_ctx *ctx, int option, int value);
+
 /* --- measurement -------------------------------------------------------- */
 /* With profiling on, every kernel launch is bracketed by HIP events on the
  * context stream; fw_profile_get reports launch count and summed device

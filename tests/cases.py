@@ -40,4 +40,17 @@ def island():
     return c
 
 
-CASES = {"tiny2d": tiny2d, "tiny1d": tiny1d, "island": island}
+def tiny2d_dx15():
+    """The 2-D beach on a 1.5 m grid: spacings that are not powers of two,
+    so every division by DX/DY stays a division."""
+    c = fwcases.beach_2d(mglob=64, nglob=48, dx=1.5, nfreq=4, ndir=3, flat_depth=2.0,
+                         slope=1 / 25.0, x_toe=45.0, xc_wk=15.0, sponge=0.0)
+    c.params["Sponge_west_width"] = 9.0
+    c.params["Sponge_south_width"] = 6.0
+    c.params["Sponge_north_width"] = 6.0
+    c.spectrum["amp"] = c.spectrum["amp"] * 4.0
+    c.name = "tiny2d_dx15"
+    return c
+
+
+CASES = {"tiny2d": tiny2d, "tiny1d": tiny1d, "island": island, "tiny2d_dx15": tiny2d_dx15}

@@ -22,7 +22,8 @@ KEY = ["Eta", "U", "V", "Ubar", "Vbar", "HU", "HV", "H", "MASK", "MASK9",
        "AGE_BREAKING", "nu_break", "DepthX", "DepthY", "P", "Q"]
 PLAN = {"tiny2d": (["dp", "sp"], [1, 60, 250]),
         "tiny1d": (["dp", "sp"], [1, 400]),
-        "island": (["dp", "sp"], [1, 120])}
+        "island": (["dp", "sp"], [1, 120]),
+        "tiny2d_dx15": (["dp", "sp"], [1, 150])}
 
 
 def pack(d, keys=None):
