@@ -36,8 +36,9 @@ B_ALG_WORDS = 93            # algorithmic words per interior cell-step (whole st
 KERNEL_TRAFFIC = {
     "flux_x": (8, 2, 3, "xface"),        # U V HU HV Eta U4 V4 DepthX, MASK MASK9 -> P Fx Gx
     "flux_y": (8, 2, 3, "yface"),
-    "disp_deriv": (9, 1, 29, "cell"),    # U V U0 V0 Eta Depth P Q (+Ut) -> 26 derivatives, ETAT, Ut, Vt
-    "disp_terms": (31, 1, 10, "cell"),   # derivative arrays, Depth, Eta, U, V -> U4 V4 U1p V1p U1pp..V3
+    "disp_deriv": (6, 1, 11, "cell"),    # U V Depth Eta P Q, MASK9 -> 8 second derivatives, ETAT ETAx ETAy
+    "disp_terms": (17, 1, 10, "cell"),   # 8 second derivatives, ETAT ETAx ETAy, U V U0 V0 Depth Eta
+                                         #   -> U4 V4 U1p V1p U1pp V1pp U2 V2 U3 V3
     "sources": (24, 2, 2, "inner"),
     "huv": (24, 2, 12, "cell"),
     "trid": (6, 0, 2, "inner"),
@@ -63,12 +64,11 @@ def parse():
 
 
 def sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    """Device-wide barrier for work queued outside the solver's stream (only
+    torch can have queued any, and only if something imported it)."""
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_available():
+        torch.cuda.synchronize()
 
 
 def kernel_traffic_bytes(name, rb, M, N, mg, ng):

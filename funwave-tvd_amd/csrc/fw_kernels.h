@@ -300,7 +300,7 @@ __global__ void k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
   if (i <= g.M - 1 && j <= g.N - 1) {
     d.ETAT[o] = -dv(d.P[IX(i + 1, j)] - d.P[IX(i, j)], dx) - dv(d.Q[IX(i, j + 1)] - d.Q[IX(i, j)], dy);
   }
-  if (g2) {  // :133-142
+  if (g2 && p.full_state) {  // :133-142
     d.Ut[o] = ddiv(d.U[o] - d.U0[o], DT);
     d.Vt[o] = ddiv(d.V[o] - d.V0[o], DT);
   }
@@ -322,13 +322,18 @@ __global__ void k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
   d.DUxy[o] = wall ? (R)0 : gfxy<R>(DU, i, j, dx, dy) * m9;
   d.DVxy[o] = wall ? (R)0 : gfxy<R>(DV, i, j, dx, dy) * m9;
   d.DVyy[o] = gfyy<R>(DV, i, j, dy) * m9;
-  if (g2) {
+  if (g2 || p.SHOW_BRK) {  // read by WAVE_BREAKING
+    d.ETAx[o] = dfx(g, d.Eta, i, j, dx) * m9;
+    d.ETAy[o] = dfy(g, d.Eta, i, j, dy) * m9;
+  }
+  // The first derivatives are used only at their own cell by the nonlinear
+  // terms, which form them from the stencil themselves; the arrays are
+  // filled only when every intermediate is kept.
+  if (g2 && p.full_state) {
     d.Ux[o] = dfx(g, U, i, j, dx) * m9;
     d.Vx[o] = dfx(g, V, i, j, dx) * m9;
     d.Uy[o] = dfy(g, U, i, j, dy) * m9;
     d.Vy[o] = dfy(g, V, i, j, dy) * m9;
-    d.ETAx[o] = dfx(g, d.Eta, i, j, dx) * m9;
-    d.ETAy[o] = dfy(g, d.Eta, i, j, dy) * m9;
     d.DUx[o] = gfx<R>(DU, i, j, dx) * m9;
     d.DVx[o] = gfx<R>(DV, i, j, dx) * m9;
     d.DUy[o] = gfy<R>(DU, i, j, dy) * m9;
@@ -345,16 +350,13 @@ __global__ void k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
     d.DVtyy[o] = gfyy<R>(DVT, i, j, dy) * m9;
     d.DUtxy[o] = wall ? (R)0 : gfxy<R>(DUT, i, j, dx, dy) * m9;
     d.DVtxy[o] = wall ? (R)0 : gfxy<R>(DVT, i, j, dx, dy) * m9;
-  } else if (p.SHOW_BRK) {
-    d.ETAx[o] = dfx(g, d.Eta, i, j, dx) * m9;
-    d.ETAy[o] = dfy(g, d.Eta, i, j, dy) * m9;
   }
 }
 
 // CAL_DISPERSION part 2: U4, V4, U1p, V1p on all cells (:293-330) and the
 // nonlinear terms U1pp..V3 on the interior (:398-503).
 template <class R>
-__global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p) {
+__global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p, R DT) {
   int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
@@ -384,6 +386,28 @@ __global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p) {
   }
   if (!(g2 > (R)0)) return;
   if (i < g.Ib || i > g.Ie || j < g.Jb || j > g.Je) return;
+  // first derivatives of U, V, Ut, Vt, DU, DV, DUt, DVt at this cell, the
+  // same expressions (and masking) as CAL_DISPERSION's arrays
+  const R m9 = (R)d.MASK9[o], mdf = p.MDF;
+  const bool wall = (i == g.Ib || i == g.Ie || j == g.Jb || j == g.Je);
+  const R *Uv = d.U, *Vv = d.V, *Dp = d.Depth, *U0 = d.U0, *V0 = d.V0;
+  auto DU = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * Uv[q]; };
+  auto DV = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * Vv[q]; };
+  auto UT = [&](int a, int b) { size_t q = IX(a, b); return ddiv(Uv[q] - U0[q], DT); };
+  auto VT = [&](int a, int b) { size_t q = IX(a, b); return ddiv(Vv[q] - V0[q], DT); };
+  auto DUT = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * ddiv(Uv[q] - U0[q], DT); };
+  auto DVT = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * ddiv(Vv[q] - V0[q], DT); };
+  const R f_Ux = dfx(g, Uv, i, j, dx) * m9, f_Vx = dfx(g, Vv, i, j, dx) * m9;
+  const R f_Uy = dfy(g, Uv, i, j, dy) * m9, f_Vy = dfy(g, Vv, i, j, dy) * m9;
+  const R f_DUx = gfx<R>(DU, i, j, dx) * m9, f_DVy = gfy<R>(DV, i, j, dy) * m9;
+  const R f_Utx = gfx<R>(UT, i, j, dx) * m9, f_Vty = gfy<R>(VT, i, j, dy) * m9;
+  const R f_Utxx = gfxx<R>(UT, i, j, dx) * m9, f_Vtyy = gfyy<R>(VT, i, j, dy) * m9;
+  const R f_Utxy = wall ? (R)0 : gfxy<R>(UT, i, j, dx, dy) * m9;
+  const R f_Vtxy = wall ? (R)0 : gfxy<R>(VT, i, j, dx, dy) * m9;
+  const R f_DUtx = gfx<R>(DUT, i, j, dx) * m9, f_DVty = gfy<R>(DVT, i, j, dy) * m9;
+  const R f_DUtxx = gfxx<R>(DUT, i, j, dx) * m9, f_DVtyy = gfyy<R>(DVT, i, j, dy) * m9;
+  const R f_DUtxy = wall ? (R)0 : gfxy<R>(DUT, i, j, dx, dy) * m9;
+  const R f_DVtxy = wall ? (R)0 : gfxy<R>(DVT, i, j, dx, dy) * m9;
 #define A_(n, a, b) d.n[IX(a, b)]
   R UxxVxy = A_(Uxx, i, j) + A_(Vxy, i, j);
   R UxyVyy = A_(Uxy, i, j) + A_(Vyy, i, j);
@@ -402,17 +426,17 @@ __global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p) {
   R rhy = dv(half(A_(Depth, i, j + 1) - A_(Depth, i, j - 1)), dy);
   R reta = A_(Eta, i, j);
   R ex = A_(ETAx, i, j), ey = A_(ETAy, i, j), et = A_(ETAT, i, j);
-  R Utx_ = A_(Utx, i, j), Vty_ = A_(Vty, i, j);
-  R u1pp = -reta * B2 * ex * B2 * (Utx_ + Vty_) - (R)0.5 * reta * reta * B2 * B2 * (A_(Utxx, i, j) + A_(Vtxy, i, j))
-           - ex * B2 * (A_(DUtx, i, j) + A_(DVty, i, j)) - reta * B2 * (A_(DUtxx, i, j) + A_(DVtxy, i, j));
-  R v1pp = -reta * B2 * ey * B2 * (Utx_ + Vty_) - (R)0.5 * reta * reta * B2 * B2 * (A_(Utxy, i, j) + A_(Vtyy, i, j))
-           - ey * B2 * (A_(DUtx, i, j) + A_(DVty, i, j)) - reta * B2 * (A_(DUtxy, i, j) + A_(DVtyy, i, j));
+  R Utx_ = f_Utx, Vty_ = f_Vty;
+  R u1pp = -reta * B2 * ex * B2 * (Utx_ + Vty_) - (R)0.5 * reta * reta * B2 * B2 * (f_Utxx + f_Vtxy)
+           - ex * B2 * (f_DUtx + f_DVty) - reta * B2 * (f_DUtxx + f_DVtxy);
+  R v1pp = -reta * B2 * ey * B2 * (Utx_ + Vty_) - (R)0.5 * reta * reta * B2 * B2 * (f_Utxy + f_Vtyy)
+           - ey * B2 * (f_DUtx + f_DVty) - reta * B2 * (f_DUtxy + f_DVtyy);
   R ken1 = B1 * ((R)1 - B1) * rh * et * B2 - B1 * B1 * reta * B2 * et * B2;
   R ken2 = B1 * ((R)1 - B1) * rh * reta * B2 - (R)0.5f * B1 * B1 * reta * reta * B2 * B2;
   R ken3 = B1 * et * B2;
   R ken4 = B1 * reta * B2;
-  u1pp = u1pp - ken1 * UxxVxy - ken2 * (A_(Utxx, i, j) + A_(Vtxy, i, j)) + ken3 * HUxxHVxy + ken4 * (A_(DUtxx, i, j) + A_(DVtxy, i, j));
-  v1pp = v1pp - ken1 * UxyVyy - ken2 * (A_(Utxy, i, j) + A_(Vtyy, i, j)) + ken3 * HUxyHVyy + ken4 * (A_(DUtxy, i, j) + A_(DVtyy, i, j));
+  u1pp = u1pp - ken1 * UxxVxy - ken2 * (f_Utxx + f_Vtxy) + ken3 * HUxxHVxy + ken4 * (f_DUtxx + f_DVtxy);
+  v1pp = v1pp - ken1 * UxyVyy - ken2 * (f_Utxy + f_Vtyy) + ken3 * HUxyHVyy + ken4 * (f_DUtxy + f_DVtyy);
   d.U1pp[o] = u1pp;
   d.V1pp[o] = v1pp;
   R omB = (R)1 - B1;
@@ -422,8 +446,8 @@ __global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p) {
   ken4 = ((R)0.5 * omB * omB * rh * rh * B2 * B2 - B1 * omB * rh * reta * B2 + (R)0.5 * (B1 * B1 - (R)1) * reta * reta * B2 * B2);
   R ken5 = (omB * omB * rh * rhy * B2 * B2 - B1 * omB * (rhy * reta * B2 + rh * ey * B2) + (B1 * B1 - (R)1) * reta * ey * B2 * B2);
   R u = A_(U, i, j), v = A_(V, i, j);
-  R ux = A_(Ux, i, j), vx = A_(Vx, i, j), uy = A_(Uy, i, j), vy = A_(Vy, i, j);
-  R dux = A_(DUx, i, j), dvy = A_(DVy, i, j);
+  R ux = f_Ux, vx = f_Vx, uy = f_Uy, vy = f_Vy;
+  R dux = f_DUx, dvy = f_DVy;
   d.U2[o] = ken1 * (u * HUxxHVxy + v * HUxyHVyy) + ken2 * (ux * HUxxHVxy + u * HUxxHVxy_x + vx * HUxyHVyy + v * HUxyHVyy_x)
             + ken3 * (u * UxxVxy + v * UxyVyy) + ken4 * (ux * UxxVxy + u * UxxVxy_x + vx * UxyVyy + v * UxyVyy_x)
             + B2 * B2 * (dux + dvy + reta * B2 * (ux + vy)) * (HUxxHVxy + ex * B2 * (ux + vy) + reta * B2 * UxxVxy);
@@ -515,8 +539,9 @@ __device__ __forceinline__ R hydro(R xi, R de, R g3) {
   return (R)0.5 * GRAV * (xi * xi * g3 + (R)2 * xi * de);
 }
 
-// x faces: states, wave speeds (WAVE_SPEED :352-439), HLL (:150-194) and
-// BOUNDARY_CONDITION (bc.F:53-295) resolved per face.
+// Wave speeds (WAVE_SPEED fluxes.F:352-439) of one computed face from its own
+// per-face reconstruction; ghost faces copy the speeds of the nearest
+// computed face and use these.
 template <class R>
 __device__ __forceinline__ void xface_speed(const Dev<R> &d, const Geo &g, const Prm<R> &p, int i, int j, R &sl, R &sr) {
   size_t o = IX(i, j);
@@ -544,39 +569,62 @@ __device__ __forceinline__ void yface_speed(const Dev<R> &d, const Geo &g, const
   sr = rmax(vr + spr, us + sps);
 }
 
+// x faces (fluxes.F:52-117 with CONSTRUCT_HO_X :1531-1640, WAVE_SPEED
+// :352-439, HLL :150-194) and BOUNDARY_CONDITION (bc.F:53-295) resolved per
+// face.
+//
+// The reconstruction is organised by cell rather than by face.  The limited
+// slope D(k) of face k (din_val on cells k-2..k+1) is needed by faces k-1, k
+// and k+1, and the van Leer weights of the pair (D(c), D(c+1)) give both the
+// right state of face c and the left state of face c+1.  Each is evaluated
+// once, from the same operands in the same order as the per-face form, so
+// the states are bit-identical to it.
+//
+// One wavefront = 64 consecutive cells c of one row; lane l holds D(c) and
+// the pair of cell c, and takes D(c+1) and the left state of face c from its
+// neighbours with a lane shuffle.  Lanes 1..62 own faces; lanes 0 and 63 only
+// supply neighbours, so consecutive wavefronts overlap by two cells.
+constexpr int kFluxXFaces = 62;
+
 template <class R>
-__global__ void k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+__device__ __forceinline__ void pair_states(R vc, R D, R Dn, R &Lnext, R &Rc) {
+  R w1, w2, a, b;
+  vanleer(D, Dn, w1, w2, a, b);
+  Lnext = vc + ((R)1 / (R)6) * (w1 * a + (R)2 * w2 * b);
+  Rc = vc - ((R)1 / (R)6) * ((R)2 * w1 * a + w2 * b);
+}
+
+// left/right states of face c along x for field v: lanes with dok hold a
+// valid D(c), lanes with pok a valid pair for cell c
+template <class R>
+__device__ __forceinline__ void recon_x(const R *v, size_t o, const int (&mk)[4], bool dok, bool pok, R &L, R &Rf) {
+  R D = (R)0, vc = (R)0;
+  if (dok) {
+    R vm2 = v[o - 2], vm1 = v[o - 1], v0 = v[o], vp1 = v[o + 1];
+    D = din_val(vm2, vm1, v0, vp1, mk[0], mk[1], mk[2], mk[3]);
+    vc = v0;
+  }
+  R Dn = __shfl_down(D, 1);
+  R Ln = (R)0, Rc = (R)0;
+  if (pok) pair_states(vc, D, Dn, Ln, Rc);
+  L = __shfl_up(Ln, 1);
+  Rf = Rc;
+}
+
+// Everything after the states: P, F, G, the wave speeds, HLL and the wall
+// and dry-cell conditions of one x face.
+template <class R>
+__device__ __forceinline__ void xface_finish(const Dev<R> &d, const Geo &g, const Prm<R> &p, int i, int j, bool comp,
+                                             R uL, R uR, R vL, R vR, R huL, R huR, R hvL, R hvR, R eL, R eR, R u4L,
+                                             R u4R, R v4L, R v4R) {
+  const size_t o = IX(i, j);
   const int M1 = g.M + 1;
-  if (i > M1 || j > g.N) return;
-  size_t o = IX(i, j);
   const R g1 = p.G1, g3 = p.G3;
-  const bool comp = (i >= g.Ib && i <= g.Ie + 1 && j >= g.Jb && j <= g.Je);
-  R uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R;
-  if (comp) {
-    muscl(d.U, d.MASK, o, 1, uL, uR);
-    muscl(d.V, d.MASK, o, 1, vL, vR);
-    muscl(d.HU, d.MASK, o, 1, huL, huR);
-    muscl(d.HV, d.MASK, o, 1, hvL, hvR);
-    muscl(d.Eta, d.MASK, o, 1, eL, eR);
-    if (p.full_state) {
-      d.UxL[o] = uL; d.UxR[o] = uR; d.VxL[o] = vL; d.VxR[o] = vR;
-      d.HUxL[o] = huL; d.HUxR[o] = huR; d.HVxL[o] = hvL; d.HVxR[o] = hvR;
-      d.EtaRxL[o] = eL; d.EtaRxR[o] = eR;
-    }
-    if (p.DISPERSION) {
-      muscl(d.U4, d.MASK, o, 1, u4L, u4R);
-      muscl(d.V4, d.MASK, o, 1, v4L, v4R);
-      if (p.full_state) { d.U4xL[o] = u4L; d.U4xR[o] = u4R; d.V4xL[o] = v4L; d.V4xR[o] = v4R; }
-    } else {
-      u4L = d.U4xL[o]; u4R = d.U4xR[o]; v4L = d.V4xL[o]; v4R = d.V4xR[o];
-    }
-  } else {
-    uL = d.UxL[o]; uR = d.UxR[o]; vL = d.VxL[o]; vR = d.VxR[o];
-    huL = d.HUxL[o]; huR = d.HUxR[o]; hvL = d.HVxL[o]; hvR = d.HVxR[o];
-    eL = d.EtaRxL[o]; eR = d.EtaRxR[o];
-    u4L = d.U4xL[o]; u4R = d.U4xR[o]; v4L = d.V4xL[o]; v4R = d.V4xR[o];
+  if (comp && p.full_state) {
+    d.UxL[o] = uL; d.UxR[o] = uR; d.VxL[o] = vL; d.VxR[o] = vR;
+    d.HUxL[o] = huL; d.HUxR[o] = huR; d.HVxL[o] = hvL; d.HVxR[o] = hvR;
+    d.EtaRxL[o] = eL; d.EtaRxR[o] = eR;
+    if (p.DISPERSION) { d.U4xL[o] = u4L; d.U4xR[o] = u4R; d.V4xL[o] = v4L; d.V4xR[o] = v4R; }
   }
   const R dpx = d.DepthX[o];
   R hL = eL + dpx, hR = eR + dpx;
@@ -628,40 +676,60 @@ __global__ void k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
   d.P[o] = P; d.Fx[o] = Fx; d.Gx[o] = Gx;
 }
 
-template <class R>
-__global__ void k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int N1 = g.N + 1;
-  if (i > g.M || j > N1) return;
-  size_t o = IX(i, j);
-  const ptrdiff_t s = (ptrdiff_t)g.P;
-  const R g1 = p.G1, g3 = p.G3;
-  const bool comp = (i >= g.Ib && i <= g.Ie && j >= g.Jb && j <= g.Je + 1);
+// blockDim (64, 4): threadIdx.x is the lane, threadIdx.y picks the row.
+template <class R, bool DISP>
+__global__ void __launch_bounds__(256) k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
+  const int lane = threadIdx.x;
+  const int c = blockIdx.x * kFluxXFaces + lane;  // cell / face index of this lane
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (j > g.N) return;  // whole wavefront
+  const bool own = lane >= 1 && lane <= kFluxXFaces && c <= g.M + 1;
+  const bool row = j >= g.Jb && j <= g.Je;
+  const bool comp = row && c >= g.Ib && c <= g.Ie + 1;
+  const size_t o = IX(c < 1 ? 1 : c, j);
   R uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R;
-  if (comp) {
-    muscl(d.U, d.MASK, o, s, uL, uR);
-    muscl(d.V, d.MASK, o, s, vL, vR);
-    muscl(d.HV, d.MASK, o, s, hvL, hvR);
-    muscl(d.HU, d.MASK, o, s, huL, huR);
-    muscl(d.Eta, d.MASK, o, s, eL, eR);
-    if (p.full_state) {
-      d.UyL[o] = uL; d.UyR[o] = uR; d.VyL[o] = vL; d.VyR[o] = vR;
-      d.HUyL[o] = huL; d.HUyR[o] = huR; d.HVyL[o] = hvL; d.HVyR[o] = hvR;
-      d.EtaRyL[o] = eL; d.EtaRyR[o] = eR;
+  if (row) {  // uniform over the wavefront
+    // D(c) needs cells c-2..c+1, the pair of cell c needs D(c), D(c+1)
+    const bool dok = c >= g.Ib - 1 && c <= g.Ie + 2;
+    const bool pok = c >= g.Ib - 1 && c <= g.Ie + 1;
+    int mk[4] = {0, 0, 0, 0};
+    if (dok) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) mk[k] = d.MASK[o + k - 2];
     }
-    if (p.DISPERSION) {
-      muscl(d.V4, d.MASK, o, s, v4L, v4R);
-      muscl(d.U4, d.MASK, o, s, u4L, u4R);
-      if (p.full_state) { d.U4yL[o] = u4L; d.U4yR[o] = u4R; d.V4yL[o] = v4L; d.V4yR[o] = v4R; }
-    } else {
-      u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
+    recon_x(d.U, o, mk, dok, pok, uL, uR);
+    recon_x(d.V, o, mk, dok, pok, vL, vR);
+    recon_x(d.HU, o, mk, dok, pok, huL, huR);
+    recon_x(d.HV, o, mk, dok, pok, hvL, hvR);
+    recon_x(d.Eta, o, mk, dok, pok, eL, eR);
+    if (DISP) {
+      recon_x(d.U4, o, mk, dok, pok, u4L, u4R);
+      recon_x(d.V4, o, mk, dok, pok, v4L, v4R);
     }
-  } else {
-    uL = d.UyL[o]; uR = d.UyR[o]; vL = d.VyL[o]; vR = d.VyR[o];
-    huL = d.HUyL[o]; huR = d.HUyR[o]; hvL = d.HVyL[o]; hvR = d.HVyR[o];
-    eL = d.EtaRyL[o]; eR = d.EtaRyR[o];
-    u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
+  }
+  if (!own) return;
+  if (!comp) {
+    uL = d.UxL[o]; uR = d.UxR[o]; vL = d.VxL[o]; vR = d.VxR[o];
+    huL = d.HUxL[o]; huR = d.HUxR[o]; hvL = d.HVxL[o]; hvR = d.HVxR[o];
+    eL = d.EtaRxL[o]; eR = d.EtaRxR[o];
+  }
+  if (!comp || !DISP) { u4L = d.U4xL[o]; u4R = d.U4xR[o]; v4L = d.V4xL[o]; v4R = d.V4xR[o]; }
+  xface_finish(d, g, p, c, j, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
+}
+
+// y faces (CONSTRUCT_HO_Y fluxes.F:1642-1754, bc.F:147-210 and :221-293).
+template <class R>
+__device__ __forceinline__ void yface_finish(const Dev<R> &d, const Geo &g, const Prm<R> &p, int i, int j, bool comp,
+                                             R uL, R uR, R vL, R vR, R huL, R huR, R hvL, R hvR, R eL, R eR, R u4L,
+                                             R u4R, R v4L, R v4R) {
+  const size_t o = IX(i, j);
+  const int N1 = g.N + 1;
+  const R g1 = p.G1, g3 = p.G3;
+  if (comp && p.full_state) {
+    d.UyL[o] = uL; d.UyR[o] = uR; d.VyL[o] = vL; d.VyR[o] = vR;
+    d.HUyL[o] = huL; d.HUyR[o] = huR; d.HVyL[o] = hvL; d.HVyR[o] = hvR;
+    d.EtaRyL[o] = eL; d.EtaRyR[o] = eR;
+    if (p.DISPERSION) { d.U4yL[o] = u4L; d.U4yR[o] = u4R; d.V4yL[o] = v4L; d.V4yR[o] = v4R; }
   }
   const R dpy = d.DepthY[o];
   R hL = eL + dpy, hR = eR + dpy;
@@ -708,6 +776,109 @@ __global__ void k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
     }
   }
   d.Q[o] = Q; d.Fy[o] = Fy; d.Gy[o] = Gy;
+}
+
+template <class R>
+__device__ __forceinline__ void yface_stored(const Dev<R> &d, const Geo &g, const Prm<R> &p, int i, int j) {
+  const size_t o = IX(i, j);
+  yface_finish(d, g, p, i, j, false, d.UyL[o], d.UyR[o], d.VyL[o], d.VyR[o], d.HUyL[o], d.HUyR[o], d.HVyL[o],
+               d.HVyR[o], d.EtaRyL[o], d.EtaRyR[o], d.U4yL[o], d.U4yR[o], d.V4yL[o], d.V4yR[o]);
+}
+
+// Sliding window of one field along a column: w[0..3] = v(k-2..k+1) when D(k)
+// is formed; D is the last slope, L the left state of the next face.
+template <class R>
+struct ColWin {
+  R w[4];
+  R D, L;
+};
+template <class R>
+__device__ __forceinline__ void cw_load(ColWin<R> &s, const R *v, size_t o, ptrdiff_t st) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) s.w[k] = v[o + (k - 2) * st];
+}
+template <class R>
+__device__ __forceinline__ void cw_shift(ColWin<R> &s, const R *v, size_t onew) {
+  s.w[0] = s.w[1];
+  s.w[1] = s.w[2];
+  s.w[2] = s.w[3];
+  s.w[3] = v[onew];
+}
+template <class R>
+__device__ __forceinline__ R cw_din(const ColWin<R> &s, const int (&m)[4]) {
+  return din_val(s.w[0], s.w[1], s.w[2], s.w[3], m[0], m[1], m[2], m[3]);
+}
+
+// One thread per column i and a segment of kFluxYSeg faces in j.  Along the
+// segment the thread keeps the last four cells of every field in registers,
+// forms D(j+1) once per face and the pair of cell j once per face; the first
+// computed face of a segment costs one extra slope and pair.
+constexpr int kFluxYSeg = 16;
+
+template <class R, bool DISP>
+__global__ void __launch_bounds__(256) k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j0 = 1 + (blockIdx.y * blockDim.y + threadIdx.y) * kFluxYSeg;
+  const int N1 = g.N + 1;
+  if (i > g.M || j0 > N1) return;
+  const int j1 = j0 + kFluxYSeg - 1 < N1 ? j0 + kFluxYSeg - 1 : N1;
+  const ptrdiff_t s = (ptrdiff_t)g.P;
+  const bool col = i >= g.Ib && i <= g.Ie;
+  // computed faces of this segment: [ja, jb]
+  int ja = j0 > g.Jb ? j0 : g.Jb;
+  int jb = j1 < g.Je + 1 ? j1 : g.Je + 1;
+  if (!col) { ja = j1 + 1; jb = j1; }
+  for (int j = j0; j <= j1 && j < ja; j++) yface_stored(d, g, p, i, j);
+  if (ja <= jb) {
+    // window for D(ja-1): cells ja-3..ja
+    int m[4];
+    size_t ob = IX(i, ja - 1);
+#pragma unroll
+    for (int k = 0; k < 4; k++) m[k] = d.MASK[ob + (k - 2) * s];
+    constexpr int NV = DISP ? 7 : 5;
+    const R *fv[7] = {d.U, d.V, d.HU, d.HV, d.Eta, d.U4, d.V4};
+    ColWin<R> w[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      cw_load(w[q], fv[q], ob, s);
+      w[q].D = cw_din(w[q], m);
+    }
+    // D(ja) and the pair of cell ja-1 -> left state of face ja
+    size_t on = IX(i, ja + 1);
+    m[0] = m[1]; m[1] = m[2]; m[2] = m[3]; m[3] = d.MASK[on];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      cw_shift(w[q], fv[q], on);
+      R Dn = cw_din(w[q], m), Rc;
+      pair_states(w[q].w[1], w[q].D, Dn, w[q].L, Rc);
+      w[q].D = Dn;
+    }
+    for (int j = ja; j <= jb; j++) {
+      on = IX(i, j + 2);
+      m[0] = m[1]; m[1] = m[2]; m[2] = m[3]; m[3] = d.MASK[on];
+      R Lf[NV], Rf[NV];
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        cw_shift(w[q], fv[q], on);
+        R Dn = cw_din(w[q], m), Ln;
+        pair_states(w[q].w[1], w[q].D, Dn, Ln, Rf[q]);
+        Lf[q] = w[q].L;
+        w[q].L = Ln;
+        w[q].D = Dn;
+      }
+      R u4L, u4R, v4L, v4R;
+      if (DISP) {
+        u4L = Lf[5 % NV]; u4R = Rf[5 % NV]; v4L = Lf[6 % NV]; v4R = Rf[6 % NV];
+      } else {
+        const size_t o = IX(i, j);
+        u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
+      }
+      yface_finish(d, g, p, i, j, true, Lf[0], Rf[0], Lf[1], Rf[1], Lf[2], Rf[2], Lf[3], Rf[3], Lf[4], Rf[4], u4L,
+                   u4R, v4L, v4R);
+    }
+  }
+  for (int j = (jb + 1 > j0 ? jb + 1 : j0); j <= j1; j++)
+    if (j >= ja) yface_stored(d, g, p, i, j);
 }
 
 // --------------------------------------------------------------------------
@@ -951,14 +1122,21 @@ __global__ void k_trid(Dev<R> d, Geo g, Prm<R> p, int nxblk) {
 // tile while lanes 0..L-1 of wave 0 run the recurrence on the current chunk
 // out of LDS.  The forward sweep writes the modified (c', d') back in place
 // (every element is loaded one chunk before it is overwritten), the backward
-// sweep streams them back in reverse.  The arithmetic per element is the
-// reference's (tridiagnal.F:383-396), so results are bitwise the same as
-// the one-thread-per-line sweep.
+// sweep streams them back in reverse.
+//
+// The elimination step of tridiagnal.F:383-396 is
+//   c' = (c/a) / (1/a - c'_prev),  d' = (d/a - d'_prev) / (1/a - c'_prev)
+// (rows with a == 0, and the first row, keep c, d).  1/a, c/a and d/a do not
+// depend on the previous row, so the 256 threads form them while staging a
+// chunk and the serial lanes do only the two dependent divisions.  Every
+// quantity is the same correctly rounded operation on the same operands, so
+// results are bitwise those of the one-thread-per-line sweep.
 template <class R, int L, int CH>
 struct TridTile {
-  R a[2][L][CH + 1];
-  R c[2][L][CH + 1];
-  R d[2][L][CH + 1];
+  R a[2][L][CH + 1];  // forward: 1/a;  backward: solution
+  R c[2][L][CH + 1];  // forward: c/a, then c';  backward: c'
+  R d[2][L][CH + 1];  // forward: d/a, then d';  backward: d'
+  unsigned char keep[2][L][CH + 1];  // forward: row keeps its c, d
 };
 
 template <class R, int L, int CH>
@@ -971,10 +1149,8 @@ __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
   const int line0 = blk * L;
   const int nl = min(L, nlines_tot - line0);
   const R *A = xdir ? d.xa : d.ya;
-  const R *C = xdir ? d.xc : d.yc;
-  const R *D = xdir ? d.xd : d.yd;
-  R *CP = (R *)C;
-  R *DP = (R *)D;
+  R *CP = xdir ? d.xc : d.yc;
+  R *DP = xdir ? d.xd : d.yd;
   R *Z = xdir ? d.U : d.V;
   const int t = threadIdx.x;
   constexpr int PER = (L * CH + 255) / 256;
@@ -1002,33 +1178,55 @@ __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
       }
     }
   };
-  auto regs_to_lds = [&](int b) {
+  // forward staging: normalised row (1/a, c/a, d/a) or the kept (c, d)
+  auto stage_fwd = [&](int ch, int b) {
 #pragma unroll
     for (int q = 0; q < PER; q++) {
       int e = t + q * 256, l, k;
       emap(e, l, k);
-      if (e < L * CH) { T.a[b][l][k] = ra[q]; T.c[b][l][k] = rc[q]; T.d[b][l][k] = rd[q]; }
+      if (e < L * CH) {
+        const R a = ra[q];
+        const bool keep = (ch * CH + k == 0) || !(a != (R)0);
+        T.keep[b][l][k] = keep;
+        if (keep) {
+          T.c[b][l][k] = rc[q];
+          T.d[b][l][k] = rd[q];
+        } else {
+          T.a[b][l][k] = ddiv((R)1, a);
+          T.c[b][l][k] = ddiv(rc[q], a);
+          T.d[b][l][k] = ddiv(rd[q], a);
+        }
+      }
+    }
+  };
+  auto stage_bwd = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      int e = t + q * 256, l, k;
+      emap(e, l, k);
+      if (e < L * CH) { T.c[b][l][k] = rc[q]; T.d[b][l][k] = rd[q]; }
     }
   };
   // ---- forward sweep
   R cprev = (R)0, dprev = (R)0;
-  load_regs(0, A, C, D);
-  regs_to_lds(0);
+  load_regs(0, A, CP, DP);
+  stage_fwd(0, 0);
   __syncthreads();
   for (int ch = 0; ch < nch; ch++) {
     const int b = ch & 1;
-    if (ch + 1 < nch) load_regs(ch + 1, A, C, D);
+    if (ch + 1 < nch) load_regs(ch + 1, A, CP, DP);
     if (t < nl) {
       const int kmax = min(CH, n - ch * CH);
+#pragma unroll 4
       for (int k = 0; k < kmax; k++) {
-        R a = T.a[b][t][k], c = T.c[b][t][k], dd = T.d[b][t][k];
-        if (ch * CH + k > 0 && a != (R)0) {
-          R den = ddiv((R)1, a) - cprev;
-          c = ddiv(ddiv(c, a), den);
-          dd = ddiv(ddiv(dd, a) - dprev, ddiv((R)1, a) - cprev);
+        R c = T.c[b][t][k], dd = T.d[b][t][k];
+        if (!T.keep[b][t][k]) {
+          const R den = T.a[b][t][k] - cprev;
+          c = ddiv(c, den);
+          dd = ddiv(dd - dprev, den);
+          T.c[b][t][k] = c;
+          T.d[b][t][k] = dd;
         }
-        T.c[b][t][k] = c;
-        T.d[b][t][k] = dd;
         cprev = c;
         dprev = dd;
       }
@@ -1045,19 +1243,20 @@ __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
         DP[o] = T.d[b][l][k];
       }
     }
-    if (ch + 1 < nch) regs_to_lds(b ^ 1);
+    if (ch + 1 < nch) stage_fwd(ch + 1, b ^ 1);
     __syncthreads();
   }
   // ---- backward sweep, chunks in reverse
   R z = (R)0;
   load_regs(nch - 1, nullptr, CP, DP);
-  regs_to_lds((nch - 1) & 1);
+  stage_bwd((nch - 1) & 1);
   __syncthreads();
   for (int ch = nch - 1; ch >= 0; ch--) {
     const int b = ch & 1;
     if (ch - 1 >= 0) load_regs(ch - 1, nullptr, CP, DP);
     if (t < nl) {
       const int kmax = min(CH, n - ch * CH);
+#pragma unroll 4
       for (int k = kmax - 1; k >= 0; k--) {
         const int kk = ch * CH + k;
         R dd = T.d[b][t][k];
@@ -1073,7 +1272,7 @@ __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
       int kk = ch * CH + k;
       if (e < L * CH && l < nl && kk < n) Z[gix(l, kk)] = T.a[b][l][k];
     }
-    if (ch - 1 >= 0) regs_to_lds(b ^ 1);
+    if (ch - 1 >= 0) stage_bwd(b ^ 1);
     __syncthreads();
   }
 }

@@ -395,7 +395,9 @@ int do_dispersion(fw_ctx *c) {
   HaloList L{}, Lf{};
   hadd(L, d.Uxx, 2); hadd(L, d.DUxx, 2); hadd(L, d.Vyy, 3); hadd(L, d.DVyy, 3);
   hadd(L, d.Uxy, 1); hadd(L, d.DUxy, 1); hadd(L, d.Vxy, 1); hadd(L, d.DVxy, 1);
-  if (c->cfg.Gamma2 > 0) {
+  if (c->cfg.Gamma2 > 0 && !c->prm<R>().full_state) {
+    hadd(L, d.ETAx, 2); hadd(L, d.ETAy, 3);
+  } else if (c->cfg.Gamma2 > 0) {
     hadd(L, d.Ut, 2); hadd(L, d.Vt, 3); hadd(L, d.Utx, 1); hadd(L, d.Vty, 1);
     hadd(L, d.Utxx, 2); hadd(L, d.Vtyy, 3); hadd(L, d.Utxy, 1); hadd(L, d.Vtxy, 1);
     hadd(L, d.DUtxx, 2); hadd(L, d.DVtyy, 3); hadd(L, d.DUtxy, 1); hadd(L, d.DVtxy, 1);
@@ -404,7 +406,7 @@ int do_dispersion(fw_ctx *c) {
     hadd(L, d.ETAx, 2); hadd(L, d.ETAy, 3);
   }
   halo_exchange_list<R>(c, L, Lf);
-  launch(c, "disp_terms", k_disp_terms<R>, grid2(g.M, g.N, kB2), kB2, d, g, c->prm<R>());
+  launch(c, "disp_terms", k_disp_terms<R>, grid2(g.M, g.N, kB2), kB2, d, g, c->prm<R>(), (R)c->dt);
   return FW_OK;
 }
 
@@ -412,8 +414,16 @@ template <class R>
 int do_fluxes(fw_ctx *c) {
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
-  launch(c, "flux_x", k_flux_x<R>, grid2(g.M + 1, g.N, kB2), kB2, d, g, c->prm<R>());
-  launch(c, "flux_y", k_flux_y<R>, grid2(g.M, g.N + 1, kB2), kB2, d, g, c->prm<R>());
+  const dim3 gx((g.M + 1 + kFluxXFaces - 1) / kFluxXFaces, (g.N + kB2.y - 1) / kB2.y);
+  const int nseg = (g.N + 1 + kFluxYSeg - 1) / kFluxYSeg;
+  const dim3 gy((g.M + kB2.x - 1) / kB2.x, (nseg + kB2.y - 1) / kB2.y);
+  if (c->cfg.DISPERSION) {
+    launch(c, "flux_x", k_flux_x<R, true>, gx, kB2, d, g, c->prm<R>());
+    launch(c, "flux_y", k_flux_y<R, true>, gy, kB2, d, g, c->prm<R>());
+  } else {
+    launch(c, "flux_x", k_flux_x<R, false>, gx, kB2, d, g, c->prm<R>());
+    launch(c, "flux_y", k_flux_y<R, false>, gy, kB2, d, g, c->prm<R>());
+  }
   return FW_OK;
 }
 

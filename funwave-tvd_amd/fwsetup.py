@@ -101,6 +101,36 @@ def _sponge(M, N, dx, dy, widths, r_sp, a_sp):
     return sp
 
 
+def _solitary_coeffs(a0, h1, rdt):
+    """SUB_SLTRY (samples.F:582-634) in the working precision: phase speed and
+    the coefficients of u = au sech^2, eta = ae1 sech^2 + ae2 sech^4.  alpha
+    is the default-REAL literal -0.39 of INITIAL_SOLITARY_WAVE (:531)."""
+    one, two, three = rdt(1), rdt(2), rdt(3)
+    a0, h1 = rdt(a0), rdt(h1)
+    alp = rdt(np.float32(-0.39))
+    alp2 = alp + one / three
+    eps = a0 / h1
+    p = -(alp2 + two * alp * (one + eps)) / (two * alp)
+    q = eps * alp2 / alp
+    r = alp2 / (two * alp)
+    x = rdt(1.2)
+    for _ in range(11):  # Newton for the root x > 1 of x^3 + p x^2 + q x + r
+        fx = r + x * (q + x * (p + x))
+        fpx = q + x * (two * p + three * x)
+        x = x - fx / fpx
+        if not abs(fx) >= rdt(np.float32(1e-5)):
+            break
+    else:
+        raise ValueError("no solitary wave solution for AMP/DEP = %g" % (a0 / h1))
+    rx = np.sqrt(x)
+    cph = np.sqrt(rdt(GRAV) * h1)
+    au = (x - one) / (eps * rx) * cph * eps
+    bue = np.sqrt((x - one) / (rdt(4) * (alp2 - alp * x))) / h1
+    ae1 = (x - one) / (eps * three * (alp2 - alp * x)) * a0
+    ae2 = -((x - one) / (two * eps) * (x - one) * (two * alp * x + alp2) / (x * (alp2 - alp * x)) * a0)
+    return rx * cph, bue, ae1, ae2, au
+
+
 def build_state(case, precision: str = "sp") -> dict:
     """Initial state (reference names, arrays indexed [j, i]) of a case."""
     rdt = np.float64 if precision == "dp" else np.float32
@@ -167,7 +197,10 @@ def build_state(case, precision: str = "sp") -> dict:
     wk = st["WaveMaker"].strip()
     delta = float(p.get("Delta_WK", 0.5))
     st.update(Xc_WK=float(p.get("Xc_WK", 0.0)), Yc_WK=float(p.get("Yc_WK", 0.0)),
-              Ywidth_WK=float(p.get("Ywidth_WK", LARGE)), Width_WK=0.0, D_gen=0.0, Beta_gen=0.0,
+              # read (default LARGE) only for the WK_ makers (io.F); otherwise a
+              # zero module variable, which leaves the source band empty
+              Ywidth_WK=float(p.get("Ywidth_WK", LARGE)) if wk.startswith("WK_") else 0.0,
+              Width_WK=0.0, D_gen=0.0, Beta_gen=0.0,
               rlamda=0.0, Tperiod=float(p.get("Tperiod", 0.0)), Time_ramp=float(p.get("Time_ramp", 0.0)),
               FreqPeak=0.0, NumFreq=0)
     if wk.startswith("WK_REG"):
@@ -194,10 +227,8 @@ def build_state(case, precision: str = "sp") -> dict:
         st.update(Width_WK=delta * wl_peak / 2.0, FreqPeak=1.0 / float(s["peak_period"]),
                   NumFreq=len(freq), OMGN2D=(2.0 * PI_SP * freq).astype(rdt))
         st["_wk2d"] = (d_gen, beta_gen[:, 0], rl, phase)
-    elif not (wk.startswith("INI_") or wk.startswith("NONE") or wk.startswith("N_WAVE")):
+    elif not (wk.startswith("INI_SOL") or wk.startswith("NONE")):
         raise NotImplementedError(f"wavemaker {wk}")
-    if wk.startswith("INI_SOL"):
-        raise NotImplementedError("INI_SOL initial solitary wave")
     xmk = np.zeros(M)
     ymk = np.zeros(N)
     xmk[ib - 1:ie] = (np.arange(ib, ie + 1) - ib) * dx
@@ -237,11 +268,28 @@ def build_state(case, precision: str = "sp") -> dict:
     else:
         st["SPONGE"] = np.ones((N, M))
 
-    # still water, masks and depths at dry cells, init.F:1000-1025, 1143-1219
+    # initial surface and velocity: still water, or the solitary wave of
+    # INITIAL_SOLITARY_WAVE (init.F:907-910, samples.F:511-571) over all local
+    # cells, ghosts included, centred where I - 1 = XWAVEMAKER/DX
     eta = np.zeros((N, M))
+    u0 = np.zeros((N, M))
+    if wk.startswith("INI_SOL"):
+        _, bue, ae1, ae2, au = _solitary_coeffs(float(p["AMP"]), float(p["DEP"]), rdt)
+        I = np.arange(1, M + 1).astype(rdt)
+        sc = rdt(1) / np.cosh(bue * (I - rdt(float(p["XWAVEMAKER"])) / rdt(dx) - rdt(1)) * rdt(dx))
+        e1 = ae1 * sc * sc + ae2 * sc * sc * sc * sc
+        u1 = au * sc * sc
+        if not bool(p.get("SolitaryPositiveDirection", True)):
+            u1 = -u1
+        eta[:, :] = e1.astype(np.float64)[None, :]
+        u0[:, :] = u1.astype(np.float64)[None, :]
+    # masks and depths at dry cells, init.F:1000-1025, 1143-1219
     mask = np.where(eta < -depth, 0, 1).astype(np.int32)
     eta = np.where(mask == 0, -mindep - depth, eta)
     h = np.maximum(eta * st["Gamma3"] + depth, mindep)
+    # HU = H U, Ubar = HU + gamma1 U1p H (init.F:1017-1026); U1p is zero there
+    # because CAL_DISPERSION runs before MASK9 is formed (:1158-1163)
+    hu = h * u0
     mask9 = np.zeros((N, M), np.int32)
     m = mask
     prod = np.ones((N - 2, M - 2), np.int32)
@@ -274,6 +322,7 @@ def build_state(case, precision: str = "sp") -> dict:
             fy[g, :] = np.where(b, depth[g - 1, :], fy[g, :])
     st.update(Depth=depth.astype(rdt), DepthX=fx.astype(rdt), DepthY=fy.astype(rdt),
               Eta=eta.astype(rdt), H=h.astype(rdt), MASK=mask, MASK9=mask9,
+              U=u0.astype(rdt), HU=hu.astype(rdt), Ubar=hu.astype(rdt),
               MASK_STRUC=np.ones((N, M), np.int32))
     for k in ("Cd", "SPONGE"):
         st[k] = st[k].astype(rdt)

@@ -116,7 +116,8 @@ int fw_reduce_stats(fw_ctx *ctx, fw_stats *out);
 
 /* --- options ------------------------------------------------------------ */
 /* FW_OPT_FULL_STATE = 1 keeps every intermediate array of MODULE GLOBAL up to
- * date (face states UxL..GyR, HxL.., SxL.., the wave-speed copies), for
+ * date (face states UxL..GyR, HxL.., SxL.., the wave-speed copies, and the
+ * first derivatives Ut, Vt, Ux.., Utx.., DUtxx.. of CAL_DISPERSION), for
  * routine-by-routine comparison with the reference.  Default 0: only the
  * arrays a later routine reads are written; results are identical. */
 enum { FW_OPT_FULL_STATE = 1 };

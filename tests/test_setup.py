@@ -6,7 +6,7 @@ import pytest
 from common import load_golden
 
 
-@pytest.mark.parametrize("name", ["tiny2d", "tiny1d"])
+@pytest.mark.parametrize("name", ["tiny2d", "tiny1d", "island"])
 def test_setup_matches_reference_initialisation(name):
     import cases
     import fwsetup
@@ -21,8 +21,9 @@ def test_setup_matches_reference_initialisation(name):
         assert st[k] == pytest.approx(ref[k], rel=1e-12, abs=1e-12), k
     for k in ("Width_WK", "D_gen", "Beta_gen", "rlamda", "FreqPeak"):
         assert st[k] == pytest.approx(ref[k], rel=1e-6, abs=1e-12), k
-    for k in ("Depth", "DepthX", "DepthY", "Eta", "H", "Cd"):
-        np.testing.assert_allclose(st[k], ref.get(k, 0 * st[k]), rtol=1e-12, atol=1e-12, err_msg=k)
+    for k in ("Depth", "DepthX", "DepthY", "Eta", "H", "Cd", "U", "HU", "Ubar"):
+        a = st.get(k, np.zeros_like(st["Depth"]))
+        np.testing.assert_allclose(a, ref.get(k, 0 * a), rtol=1e-12, atol=1e-12, err_msg=k)
     np.testing.assert_array_equal(st["MASK"], ref["MASK"])
     np.testing.assert_array_equal(st["MASK9"][3:-3, 3:-3], ref["MASK9"][3:-3, 3:-3])
     np.testing.assert_allclose(st["SPONGE"], ref["SPONGE"], rtol=1e-6)
