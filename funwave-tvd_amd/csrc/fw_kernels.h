@@ -778,107 +778,74 @@ __device__ __forceinline__ void yface_finish(const Dev<R> &d, const Geo &g, cons
   d.Q[o] = Q; d.Fy[o] = Fy; d.Gy[o] = Gy;
 }
 
-template <class R>
-__device__ __forceinline__ void yface_stored(const Dev<R> &d, const Geo &g, const Prm<R> &p, int i, int j) {
-  const size_t o = IX(i, j);
-  yface_finish(d, g, p, i, j, false, d.UyL[o], d.UyR[o], d.VyL[o], d.VyR[o], d.HUyL[o], d.HUyR[o], d.HVyL[o],
-               d.HVyR[o], d.EtaRyL[o], d.EtaRyR[o], d.U4yL[o], d.U4yR[o], d.V4yL[o], d.V4yR[o]);
-}
+// y faces: the same cell-wise reconstruction as along x, with the
+// neighbours exchanged through LDS.  A workgroup of 64 x kFluxYRows threads
+// covers 64 columns and kFluxYRows consecutive faces j of each; thread row r
+// forms D(j) and the pair of cell j, takes D(j+1) from row r+1 and the left
+// state of face j from row r-1.  Rows 1..kFluxYRows-2 own their face, so
+// consecutive workgroups overlap by two rows.
+constexpr int kFluxYRows = 16;
+constexpr int kFluxYFaces = kFluxYRows - 2;
 
-// Sliding window of one field along a column: w[0..3] = v(k-2..k+1) when D(k)
-// is formed; D is the last slope, L the left state of the next face.
 template <class R>
-struct ColWin {
-  R w[4];
-  R D, L;
-};
-template <class R>
-__device__ __forceinline__ void cw_load(ColWin<R> &s, const R *v, size_t o, ptrdiff_t st) {
-#pragma unroll
-  for (int k = 0; k < 4; k++) s.w[k] = v[o + (k - 2) * st];
+__device__ __forceinline__ void recon_y(const R *v, size_t o, ptrdiff_t s, const int (&mk)[4], bool dok, bool pok,
+                                        R (&sD)[kFluxYRows][64], R (&sL)[kFluxYRows][64], R &L, R &Rf) {
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  R D = (R)0, vc = (R)0;
+  if (dok) {
+    R vm2 = v[o - 2 * s], vm1 = v[o - s], v0 = v[o], vp1 = v[o + s];
+    D = din_val(vm2, vm1, v0, vp1, mk[0], mk[1], mk[2], mk[3]);
+    vc = v0;
+  }
+  sD[ty][tx] = D;
+  __syncthreads();
+  R Ln = (R)0, Rc = (R)0;
+  if (pok && ty + 1 < kFluxYRows) pair_states(vc, D, sD[ty + 1][tx], Ln, Rc);
+  sL[ty][tx] = Ln;
+  __syncthreads();
+  L = ty >= 1 ? sL[ty - 1][tx] : (R)0;
+  Rf = Rc;
 }
-template <class R>
-__device__ __forceinline__ void cw_shift(ColWin<R> &s, const R *v, size_t onew) {
-  s.w[0] = s.w[1];
-  s.w[1] = s.w[2];
-  s.w[2] = s.w[3];
-  s.w[3] = v[onew];
-}
-template <class R>
-__device__ __forceinline__ R cw_din(const ColWin<R> &s, const int (&m)[4]) {
-  return din_val(s.w[0], s.w[1], s.w[2], s.w[3], m[0], m[1], m[2], m[3]);
-}
-
-// One thread per column i and a segment of kFluxYSeg faces in j.  Along the
-// segment the thread keeps the last four cells of every field in registers,
-// forms D(j+1) once per face and the pair of cell j once per face; the first
-// computed face of a segment costs one extra slope and pair.
-constexpr int kFluxYSeg = 16;
 
 template <class R, bool DISP>
-__global__ void __launch_bounds__(256) k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j0 = 1 + (blockIdx.y * blockDim.y + threadIdx.y) * kFluxYSeg;
+__global__ void __launch_bounds__(64 * kFluxYRows) k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
+  __shared__ R sD[kFluxYRows][64];
+  __shared__ R sL[kFluxYRows][64];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = 1 + blockIdx.x * 64 + tx;
+  const int j = blockIdx.y * kFluxYFaces + ty;  // face index of this thread row
   const int N1 = g.N + 1;
-  if (i > g.M || j0 > N1) return;
-  const int j1 = j0 + kFluxYSeg - 1 < N1 ? j0 + kFluxYSeg - 1 : N1;
   const ptrdiff_t s = (ptrdiff_t)g.P;
   const bool col = i >= g.Ib && i <= g.Ie;
-  // computed faces of this segment: [ja, jb]
-  int ja = j0 > g.Jb ? j0 : g.Jb;
-  int jb = j1 < g.Je + 1 ? j1 : g.Je + 1;
-  if (!col) { ja = j1 + 1; jb = j1; }
-  for (int j = j0; j <= j1 && j < ja; j++) yface_stored(d, g, p, i, j);
-  if (ja <= jb) {
-    // window for D(ja-1): cells ja-3..ja
-    int m[4];
-    size_t ob = IX(i, ja - 1);
+  const bool own = ty >= 1 && ty <= kFluxYFaces && j <= N1 && i <= g.M;
+  const bool comp = col && j >= g.Jb && j <= g.Je + 1;
+  // D(j) needs cells j-2..j+1, the pair of cell j needs D(j), D(j+1)
+  const bool dok = col && j >= g.Jb - 1 && j <= g.Je + 2;
+  const bool pok = col && j >= g.Jb - 1 && j <= g.Je + 1;
+  const size_t o = IX(i <= g.M ? i : g.M, j < 1 ? 1 : (j > N1 ? N1 : j));
+  int mk[4] = {0, 0, 0, 0};
+  if (dok) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) m[k] = d.MASK[ob + (k - 2) * s];
-    constexpr int NV = DISP ? 7 : 5;
-    const R *fv[7] = {d.U, d.V, d.HU, d.HV, d.Eta, d.U4, d.V4};
-    ColWin<R> w[NV];
-#pragma unroll
-    for (int q = 0; q < NV; q++) {
-      cw_load(w[q], fv[q], ob, s);
-      w[q].D = cw_din(w[q], m);
-    }
-    // D(ja) and the pair of cell ja-1 -> left state of face ja
-    size_t on = IX(i, ja + 1);
-    m[0] = m[1]; m[1] = m[2]; m[2] = m[3]; m[3] = d.MASK[on];
-#pragma unroll
-    for (int q = 0; q < NV; q++) {
-      cw_shift(w[q], fv[q], on);
-      R Dn = cw_din(w[q], m), Rc;
-      pair_states(w[q].w[1], w[q].D, Dn, w[q].L, Rc);
-      w[q].D = Dn;
-    }
-    for (int j = ja; j <= jb; j++) {
-      on = IX(i, j + 2);
-      m[0] = m[1]; m[1] = m[2]; m[2] = m[3]; m[3] = d.MASK[on];
-      R Lf[NV], Rf[NV];
-#pragma unroll
-      for (int q = 0; q < NV; q++) {
-        cw_shift(w[q], fv[q], on);
-        R Dn = cw_din(w[q], m), Ln;
-        pair_states(w[q].w[1], w[q].D, Dn, Ln, Rf[q]);
-        Lf[q] = w[q].L;
-        w[q].L = Ln;
-        w[q].D = Dn;
-      }
-      R u4L, u4R, v4L, v4R;
-      if (DISP) {
-        u4L = Lf[5 % NV]; u4R = Rf[5 % NV]; v4L = Lf[6 % NV]; v4R = Rf[6 % NV];
-      } else {
-        const size_t o = IX(i, j);
-        u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
-      }
-      yface_finish(d, g, p, i, j, true, Lf[0], Rf[0], Lf[1], Rf[1], Lf[2], Rf[2], Lf[3], Rf[3], Lf[4], Rf[4], u4L,
-                   u4R, v4L, v4R);
-    }
+    for (int k = 0; k < 4; k++) mk[k] = d.MASK[o + (k - 2) * s];
   }
-  for (int j = (jb + 1 > j0 ? jb + 1 : j0); j <= j1; j++)
-    if (j >= ja) yface_stored(d, g, p, i, j);
+  R uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R;
+  recon_y(d.U, o, s, mk, dok, pok, sD, sL, uL, uR);
+  recon_y(d.V, o, s, mk, dok, pok, sD, sL, vL, vR);
+  recon_y(d.HU, o, s, mk, dok, pok, sD, sL, huL, huR);
+  recon_y(d.HV, o, s, mk, dok, pok, sD, sL, hvL, hvR);
+  recon_y(d.Eta, o, s, mk, dok, pok, sD, sL, eL, eR);
+  if (DISP) {
+    recon_y(d.U4, o, s, mk, dok, pok, sD, sL, u4L, u4R);
+    recon_y(d.V4, o, s, mk, dok, pok, sD, sL, v4L, v4R);
+  }
+  if (!own) return;
+  if (!comp) {
+    uL = d.UyL[o]; uR = d.UyR[o]; vL = d.VyL[o]; vR = d.VyR[o];
+    huL = d.HUyL[o]; huR = d.HUyR[o]; hvL = d.HVyL[o]; hvR = d.HVyR[o];
+    eL = d.EtaRyL[o]; eR = d.EtaRyR[o];
+  }
+  if (!comp || !DISP) { u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o]; }
+  yface_finish(d, g, p, i, j, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
 }
 
 // --------------------------------------------------------------------------
@@ -1116,27 +1083,35 @@ __global__ void k_trid(Dev<R> d, Geo g, Prm<R> p, int nxblk) {
 }
 
 // Line solves staged through LDS.  A workgroup of 256 threads owns L lines
-// of one direction.  All threads load the next chunk of (a, c, d) with
-// coalesced accesses (consecutive threads -> consecutive addresses: along a
-// row for x lines, across columns for y lines) into a double-buffered LDS
-// tile while lanes 0..L-1 of wave 0 run the recurrence on the current chunk
-// out of LDS.  The forward sweep writes the modified (c', d') back in place
-// (every element is loaded one chunk before it is overwritten), the backward
-// sweep streams them back in reverse.
+// of one direction, processed in chunks of CH rows with double-buffered LDS.
+// Wave 0 runs the recurrence (lane l < L on line l) out of LDS; waves 1-3
+// meanwhile write the previous chunk's results to HBM and stage the next
+// chunk (coalesced: consecutive threads take consecutive addresses, along a
+// row for x lines and across columns for y lines), so the serial lanes only
+// wait at one barrier per chunk.
 //
 // The elimination step of tridiagnal.F:383-396 is
 //   c' = (c/a) / (1/a - c'_prev),  d' = (d/a - d'_prev) / (1/a - c'_prev)
 // (rows with a == 0, and the first row, keep c, d).  1/a, c/a and d/a do not
-// depend on the previous row, so the 256 threads form them while staging a
-// chunk and the serial lanes do only the two dependent divisions.  Every
-// quantity is the same correctly rounded operation on the same operands, so
-// results are bitwise those of the one-thread-per-line sweep.
+// depend on the previous row, so the staging waves form them and the serial
+// lanes do only the two dependent divisions.  Every quantity is the same
+// correctly rounded operation on the same operands as in the row-by-row
+// sweep, so results are bitwise the same.  The forward results of all but
+// the last chunk go back to the c, d arrays; the backward sweep restages
+// them in reverse order and starts from the last chunk, still in LDS.
+template <class R>
+struct alignas(4 * sizeof(R)) TridRow {
+  R r, ca, da, keep;  // 1/a, c/a, d/a (or c, d when keep != 0)
+};
+template <class R>
+struct alignas(2 * sizeof(R)) TridCD {
+  R c, d;
+};
 template <class R, int L, int CH>
 struct TridTile {
-  R a[2][L][CH + 1];  // forward: 1/a;  backward: solution
-  R c[2][L][CH + 1];  // forward: c/a, then c';  backward: c'
-  R d[2][L][CH + 1];  // forward: d/a, then d';  backward: d'
-  unsigned char keep[2][L][CH + 1];  // forward: row keeps its c, d
+  TridRow<R> v[2][L][CH + 1];  // forward input
+  TridCD<R> o[2][L][CH + 1];   // forward output = backward input
+  R z[2][L][CH + 1];           // backward output
 };
 
 template <class R, int L, int CH>
@@ -1153,128 +1128,129 @@ __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
   R *DP = xdir ? d.xd : d.yd;
   R *Z = xdir ? d.U : d.V;
   const int t = threadIdx.x;
-  constexpr int PER = (L * CH + 255) / 256;
+  const bool serial = t < 64;  // wave 0
+  const int st = t - 64;       // staging thread index, 0..191
+  constexpr int NS = 192;
+  constexpr int PER = (L * CH + NS - 1) / NS;
+  const int nch = (n + CH - 1) / CH;
   // global index of (line, k)
   auto gix = [&](int l, int k) -> size_t {
     if (xdir) return (size_t)(g.Jb + line0 + l - 1) * g.P + (size_t)(g.Ib + k - 1);
     return (size_t)(g.Jb + k - 1) * g.P + (size_t)(g.Ib + line0 + l - 1);
   };
-  auto emap = [&](int e, int &l, int &k) {
+  // staging element q of this thread in chunk ch: (line, row in chunk), valid?
+  auto elem = [&](int q, int ch, int &l, int &k) -> bool {
+    const int e = st + q * NS;
     if (xdir) { l = e / CH; k = e % CH; } else { l = e % L; k = e / L; }
+    return e < L * CH && l < nl && ch * CH + k < n;
   };
-  const int nch = (n + CH - 1) / CH;
-  R ra[PER], rc[PER], rd[PER];
-  auto load_regs = [&](int ch, const R *X, const R *Y, const R *W) {
+  auto fwd_stage = [&](int ch, int b) {
+    R ra[PER], rc[PER], rd[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      int e = t + q * 256, l, k;
-      emap(e, l, k);
-      int kk = ch * CH + k;
-      if (e < L * CH && l < nl && kk < n) {
-        size_t o = gix(l, kk);
-        ra[q] = X ? X[o] : (R)0;
-        rc[q] = Y[o];
-        rd[q] = W[o];
+      int l, k;
+      if (elem(q, ch, l, k)) {
+        const size_t o = gix(l, ch * CH + k);
+        ra[q] = A[o]; rc[q] = CP[o]; rd[q] = DP[o];
       }
     }
-  };
-  // forward staging: normalised row (1/a, c/a, d/a) or the kept (c, d)
-  auto stage_fwd = [&](int ch, int b) {
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      int e = t + q * 256, l, k;
-      emap(e, l, k);
-      if (e < L * CH) {
+      int l, k;
+      if (elem(q, ch, l, k)) {
         const R a = ra[q];
-        const bool keep = (ch * CH + k == 0) || !(a != (R)0);
-        T.keep[b][l][k] = keep;
-        if (keep) {
-          T.c[b][l][k] = rc[q];
-          T.d[b][l][k] = rd[q];
+        TridRow<R> v;
+        if (ch * CH + k == 0 || !(a != (R)0)) {
+          v.r = (R)0; v.ca = rc[q]; v.da = rd[q]; v.keep = (R)1;
         } else {
-          T.a[b][l][k] = ddiv((R)1, a);
-          T.c[b][l][k] = ddiv(rc[q], a);
-          T.d[b][l][k] = ddiv(rd[q], a);
+          v.r = ddiv((R)1, a); v.ca = ddiv(rc[q], a); v.da = ddiv(rd[q], a); v.keep = (R)0;
         }
+        T.v[b][l][k] = v;
       }
     }
   };
-  auto stage_bwd = [&](int b) {
+  auto fwd_store = [&](int ch, int b) {
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      int e = t + q * 256, l, k;
-      emap(e, l, k);
-      if (e < L * CH) { T.c[b][l][k] = rc[q]; T.d[b][l][k] = rd[q]; }
+      int l, k;
+      if (elem(q, ch, l, k)) {
+        const size_t o = gix(l, ch * CH + k);
+        const TridCD<R> cd = T.o[b][l][k];
+        CP[o] = cd.c; DP[o] = cd.d;
+      }
     }
   };
+  auto bwd_stage = [&](int ch, int b) {
+    R rc[PER], rd[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      int l, k;
+      if (elem(q, ch, l, k)) { const size_t o = gix(l, ch * CH + k); rc[q] = CP[o]; rd[q] = DP[o]; }
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      int l, k;
+      if (elem(q, ch, l, k)) T.o[b][l][k] = TridCD<R>{rc[q], rd[q]};
+    }
+  };
+  auto bwd_store = [&](int ch, int b) {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      int l, k;
+      if (elem(q, ch, l, k)) Z[gix(l, ch * CH + k)] = T.z[b][l][k];
+    }
+  };
+
   // ---- forward sweep
-  R cprev = (R)0, dprev = (R)0;
-  load_regs(0, A, CP, DP);
-  stage_fwd(0, 0);
+  if (!serial) fwd_stage(0, 0);
   __syncthreads();
+  R cprev = (R)0, dprev = (R)0;
   for (int ch = 0; ch < nch; ch++) {
     const int b = ch & 1;
-    if (ch + 1 < nch) load_regs(ch + 1, A, CP, DP);
-    if (t < nl) {
-      const int kmax = min(CH, n - ch * CH);
-#pragma unroll 4
-      for (int k = 0; k < kmax; k++) {
-        R c = T.c[b][t][k], dd = T.d[b][t][k];
-        if (!T.keep[b][t][k]) {
-          const R den = T.a[b][t][k] - cprev;
-          c = ddiv(c, den);
-          dd = ddiv(dd - dprev, den);
-          T.c[b][t][k] = c;
-          T.d[b][t][k] = dd;
+    if (serial) {
+      if (t < nl) {
+        const int kmax = min(CH, n - ch * CH);
+        TridRow<R> cur = T.v[b][t][0];
+        for (int k = 0; k < kmax; k++) {
+          const TridRow<R> nxt = T.v[b][t][k + 1];  // slot CH is padding
+          const R den = cur.r - cprev;
+          R c = ddiv(cur.ca, den);
+          R dd = ddiv(cur.da - dprev, den);
+          if (cur.keep != (R)0) { c = cur.ca; dd = cur.da; }
+          T.o[b][t][k] = TridCD<R>{c, dd};
+          cprev = c;
+          dprev = dd;
+          cur = nxt;
         }
-        cprev = c;
-        dprev = dd;
       }
+    } else {
+      if (ch >= 1) fwd_store(ch - 1, b ^ 1);
+      if (ch + 1 < nch) fwd_stage(ch + 1, b ^ 1);
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < PER; q++) {  // store c', d' of this chunk
-      int e = t + q * 256, l, k;
-      emap(e, l, k);
-      int kk = ch * CH + k;
-      if (e < L * CH && l < nl && kk < n) {
-        size_t o = gix(l, kk);
-        CP[o] = T.c[b][l][k];
-        DP[o] = T.d[b][l][k];
-      }
-    }
-    if (ch + 1 < nch) stage_fwd(ch + 1, b ^ 1);
     __syncthreads();
   }
-  // ---- backward sweep, chunks in reverse
+  // ---- backward sweep, chunks in reverse; the last chunk is in LDS
   R z = (R)0;
-  load_regs(nch - 1, nullptr, CP, DP);
-  stage_bwd((nch - 1) & 1);
-  __syncthreads();
   for (int ch = nch - 1; ch >= 0; ch--) {
     const int b = ch & 1;
-    if (ch - 1 >= 0) load_regs(ch - 1, nullptr, CP, DP);
-    if (t < nl) {
-      const int kmax = min(CH, n - ch * CH);
-#pragma unroll 4
-      for (int k = kmax - 1; k >= 0; k--) {
-        const int kk = ch * CH + k;
-        R dd = T.d[b][t][k];
-        z = (kk == n - 1) ? dd : dd - T.c[b][t][k] * z;
-        T.a[b][t][k] = z;
+    if (serial) {
+      if (t < nl) {
+        const int kmax = min(CH, n - ch * CH);
+        TridCD<R> cur = T.o[b][t][kmax - 1];
+        for (int k = kmax - 1; k >= 0; k--) {
+          const TridCD<R> nxt = T.o[b][t][k > 0 ? k - 1 : 0];
+          z = (ch * CH + k == n - 1) ? cur.d : cur.d - cur.c * z;
+          T.z[b][t][k] = z;
+          cur = nxt;
+        }
       }
+    } else {
+      if (ch + 1 < nch) bwd_store(ch + 1, b ^ 1);
+      if (ch >= 1) bwd_stage(ch - 1, b ^ 1);
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-      int e = t + q * 256, l, k;
-      emap(e, l, k);
-      int kk = ch * CH + k;
-      if (e < L * CH && l < nl && kk < n) Z[gix(l, kk)] = T.a[b][l][k];
-    }
-    if (ch - 1 >= 0) stage_bwd(b ^ 1);
     __syncthreads();
   }
+  if (!serial) bwd_store(0, 0);
 }
 
 // GET_Eta_U_V_HU_HV tail (etauv_solver.F:568-593): dry cells, HU/HV, Froude

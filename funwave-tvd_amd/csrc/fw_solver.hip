@@ -415,14 +415,13 @@ int do_fluxes(fw_ctx *c) {
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
   const dim3 gx((g.M + 1 + kFluxXFaces - 1) / kFluxXFaces, (g.N + kB2.y - 1) / kB2.y);
-  const int nseg = (g.N + 1 + kFluxYSeg - 1) / kFluxYSeg;
-  const dim3 gy((g.M + kB2.x - 1) / kB2.x, (nseg + kB2.y - 1) / kB2.y);
+  const dim3 gy((g.M + 63) / 64, (g.N + 1 + kFluxYFaces - 1) / kFluxYFaces), by(64, kFluxYRows);
   if (c->cfg.DISPERSION) {
     launch(c, "flux_x", k_flux_x<R, true>, gx, kB2, d, g, c->prm<R>());
-    launch(c, "flux_y", k_flux_y<R, true>, gy, kB2, d, g, c->prm<R>());
+    launch(c, "flux_y", k_flux_y<R, true>, gy, by, d, g, c->prm<R>());
   } else {
     launch(c, "flux_x", k_flux_x<R, false>, gx, kB2, d, g, c->prm<R>());
-    launch(c, "flux_y", k_flux_y<R, false>, gy, kB2, d, g, c->prm<R>());
+    launch(c, "flux_y", k_flux_y<R, false>, gy, by, d, g, c->prm<R>());
   }
   return FW_OK;
 }
@@ -455,7 +454,7 @@ int do_huv(fw_ctx *c, int istage) {
       int nxblk = (ny + bs - 1) / bs, nyblk = (nx + bs - 1) / bs;
       launch(c, "trid", k_trid<R>, dim3(nxblk + nyblk), dim3(bs), d, g, c->prm<R>(), nxblk);
     } else {
-      constexpr int L = sizeof(R) == 4 ? 16 : 8, CH = 128;
+      constexpr int L = sizeof(R) == 4 ? 16 : 8, CH = 64;
       int nxblk = (ny + L - 1) / L, nyblk = (nx + L - 1) / L;
       launch(c, "trid", k_trid_lds<R, L, CH>, dim3(nxblk + nyblk), dim3(256), d, g, nxblk);
     }
