@@ -393,10 +393,23 @@ __global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p, R DT) {
   const R *Uv = d.U, *Vv = d.V, *Dp = d.Depth, *U0 = d.U0, *V0 = d.V0;
   auto DU = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * Uv[q]; };
   auto DV = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * Vv[q]; };
-  auto UT = [&](int a, int b) { size_t q = IX(a, b); return ddiv(Uv[q] - U0[q], DT); };
-  auto VT = [&](int a, int b) { size_t q = IX(a, b); return ddiv(Vv[q] - V0[q], DT); };
-  auto DUT = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * ddiv(Uv[q] - U0[q], DT); };
-  auto DVT = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * ddiv(Vv[q] - V0[q], DT); };
+  // Ut = (U-U0)/DT and max(Depth,MinDepthFrc) once per stencil point
+  // (Ut is not needed at (i,j+-1), Vt not at (i+-1,j)); DUt = hm*Ut
+  R hm[3][3], ut[3][3], vt[3][3];
+#pragma unroll
+  for (int dj = -1; dj <= 1; dj++) {
+#pragma unroll
+    for (int di = -1; di <= 1; di++) {
+      const size_t q = IX(i + di, j + dj);
+      hm[dj + 1][di + 1] = rmax(Dp[q], mdf);
+      ut[dj + 1][di + 1] = (dj == 0 || di != 0) ? ddiv(Uv[q] - U0[q], DT) : (R)0;
+      vt[dj + 1][di + 1] = (di == 0 || dj != 0) ? ddiv(Vv[q] - V0[q], DT) : (R)0;
+    }
+  }
+  auto UT = [&](int a, int b) { return ut[b - j + 1][a - i + 1]; };
+  auto VT = [&](int a, int b) { return vt[b - j + 1][a - i + 1]; };
+  auto DUT = [&](int a, int b) { return hm[b - j + 1][a - i + 1] * ut[b - j + 1][a - i + 1]; };
+  auto DVT = [&](int a, int b) { return hm[b - j + 1][a - i + 1] * vt[b - j + 1][a - i + 1]; };
   const R f_Ux = dfx(g, Uv, i, j, dx) * m9, f_Vx = dfx(g, Vv, i, j, dx) * m9;
   const R f_Uy = dfy(g, Uv, i, j, dy) * m9, f_Vy = dfy(g, Vv, i, j, dy) * m9;
   const R f_DUx = gfx<R>(DU, i, j, dx) * m9, f_DVy = gfy<R>(DV, i, j, dy) * m9;
@@ -787,30 +800,12 @@ __device__ __forceinline__ void yface_finish(const Dev<R> &d, const Geo &g, cons
 constexpr int kFluxYRows = 16;
 constexpr int kFluxYFaces = kFluxYRows - 2;
 
-template <class R>
-__device__ __forceinline__ void recon_y(const R *v, size_t o, ptrdiff_t s, const int (&mk)[4], bool dok, bool pok,
-                                        R (&sD)[kFluxYRows][64], R (&sL)[kFluxYRows][64], R &L, R &Rf) {
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  R D = (R)0, vc = (R)0;
-  if (dok) {
-    R vm2 = v[o - 2 * s], vm1 = v[o - s], v0 = v[o], vp1 = v[o + s];
-    D = din_val(vm2, vm1, v0, vp1, mk[0], mk[1], mk[2], mk[3]);
-    vc = v0;
-  }
-  sD[ty][tx] = D;
-  __syncthreads();
-  R Ln = (R)0, Rc = (R)0;
-  if (pok && ty + 1 < kFluxYRows) pair_states(vc, D, sD[ty + 1][tx], Ln, Rc);
-  sL[ty][tx] = Ln;
-  __syncthreads();
-  L = ty >= 1 ? sL[ty - 1][tx] : (R)0;
-  Rf = Rc;
-}
-
 template <class R, bool DISP>
 __global__ void __launch_bounds__(64 * kFluxYRows) k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
-  __shared__ R sD[kFluxYRows][64];
-  __shared__ R sL[kFluxYRows][64];
+  constexpr int NV = DISP ? 7 : 5;
+  // per field: D(j) of every thread row, then (after the pairs are formed)
+  // the left state of face j+1
+  __shared__ R sX[NV][kFluxYRows][64];
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i = 1 + blockIdx.x * 64 + tx;
   const int j = blockIdx.y * kFluxYFaces + ty;  // face index of this thread row
@@ -821,30 +816,55 @@ __global__ void __launch_bounds__(64 * kFluxYRows) k_flux_y(Dev<R> d, Geo g, Prm
   const bool comp = col && j >= g.Jb && j <= g.Je + 1;
   // D(j) needs cells j-2..j+1, the pair of cell j needs D(j), D(j+1)
   const bool dok = col && j >= g.Jb - 1 && j <= g.Je + 2;
-  const bool pok = col && j >= g.Jb - 1 && j <= g.Je + 1;
+  const bool pok = col && j >= g.Jb - 1 && j <= g.Je + 1 && ty + 1 < kFluxYRows;
   const size_t o = IX(i <= g.M ? i : g.M, j < 1 ? 1 : (j > N1 ? N1 : j));
+  const R *fv[7] = {d.U, d.V, d.HU, d.HV, d.Eta, d.U4, d.V4};
+  R w[NV][4];
   int mk[4] = {0, 0, 0, 0};
   if (dok) {
 #pragma unroll
     for (int k = 0; k < 4; k++) mk[k] = d.MASK[o + (k - 2) * s];
+#pragma unroll
+    for (int q = 0; q < NV; q++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) w[q][k] = fv[q][o + (k - 2) * s];
   }
-  R uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R;
-  recon_y(d.U, o, s, mk, dok, pok, sD, sL, uL, uR);
-  recon_y(d.V, o, s, mk, dok, pok, sD, sL, vL, vR);
-  recon_y(d.HU, o, s, mk, dok, pok, sD, sL, huL, huR);
-  recon_y(d.HV, o, s, mk, dok, pok, sD, sL, hvL, hvR);
-  recon_y(d.Eta, o, s, mk, dok, pok, sD, sL, eL, eR);
-  if (DISP) {
-    recon_y(d.U4, o, s, mk, dok, pok, sD, sL, u4L, u4R);
-    recon_y(d.V4, o, s, mk, dok, pok, sD, sL, v4L, v4R);
+  R D[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    D[q] = dok ? din_val(w[q][0], w[q][1], w[q][2], w[q][3], mk[0], mk[1], mk[2], mk[3]) : (R)0;
+    sX[q][ty][tx] = D[q];
   }
+  __syncthreads();
+  R Dn[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) Dn[q] = ty + 1 < kFluxYRows ? sX[q][ty + 1][tx] : (R)0;
+  __syncthreads();
+  R Rf[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    R Ln = (R)0, Rc = (R)0;
+    if (pok) pair_states(w[q][2], D[q], Dn[q], Ln, Rc);
+    Rf[q] = Rc;
+    sX[q][ty][tx] = Ln;
+  }
+  __syncthreads();
   if (!own) return;
+  R Lf[NV];
+#pragma unroll
+  for (int q = 0; q < NV; q++) Lf[q] = sX[q][ty - 1][tx];
+  R uL = Lf[0], uR = Rf[0], vL = Lf[1], vR = Rf[1], huL = Lf[2], huR = Rf[2], hvL = Lf[3], hvR = Rf[3];
+  R eL = Lf[4], eR = Rf[4], u4L, u4R, v4L, v4R;
   if (!comp) {
     uL = d.UyL[o]; uR = d.UyR[o]; vL = d.VyL[o]; vR = d.VyR[o];
     huL = d.HUyL[o]; huR = d.HUyR[o]; hvL = d.HVyL[o]; hvR = d.HVyR[o];
     eL = d.EtaRyL[o]; eR = d.EtaRyR[o];
   }
-  if (!comp || !DISP) { u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o]; }
+  if (DISP && comp) {
+    u4L = Lf[5 % NV]; u4R = Rf[5 % NV]; v4L = Lf[6 % NV]; v4R = Rf[6 % NV];
+  } else {
+    u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
+  }
   yface_finish(d, g, p, i, j, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
 }
 
