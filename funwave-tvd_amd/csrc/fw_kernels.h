@@ -873,12 +873,14 @@ __global__ void __launch_bounds__(64 * kFluxYRows) k_flux_y(Dev<R> d, Geo g, Prm
 // band, then SourceX/SourceY with slope, friction, dispersion, wavemaker and
 // breaking-diffusion terms.  nu_vis is a default REAL (single) array in the
 // reference (:69), so the viscosity sums are formed in single precision.
+// One interior cell.  With keep, the MODULE GLOBAL outputs (SourceX/Y,
+// FrcInsX/Y, BreakSourceX/Y, WaveMaker_Mass in the band) are written; the
+// values ESTIMATE_HUV needs are returned either way (wm: the cell's
+// WaveMaker_Mass, new in the band, else the stored one).
 template <class R>
-__global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
-  int i = g.Ib + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = g.Jb + blockIdx.y * blockDim.y + threadIdx.y;
-  if (i > g.Ie || j > g.Je) return;
-  size_t o = IX(i, j);
+__device__ __forceinline__ void source_terms(const Dev<R> &d, const Geo &g, const Prm<R> &p, R AA, int i, int j,
+                                             bool keep, R &sx_out, R &sy_out, R &wm_out) {
+  const size_t o = IX(i, j);
   const Dv<R> dx = dvx(p), dy = dvy(p);
   const R mdf = p.MDF, g1 = p.G1, g2 = p.G2;
   R wm;
@@ -894,14 +896,14 @@ __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
       for (int k = 0; k < p.NumFreq; k++) s = s + AA * (d.Cm[q + k * plane] * bb[k] + d.Sm[q + k * plane] * cc[k]);
       wm = s;
     }
-    d.WaveMaker_Mass[o] = wm;
+    if (keep) d.WaveMaker_Mass[o] = wm;
   } else {
     wm = d.WaveMaker_Mass[o];
   }
   R u = d.U[o], v = d.V[o];
   R spd = dsqrt(u * u + v * v);
   R fix = -(d.Cd[o] * u * spd);
-  d.FrcInsX[o] = fix;
+  if (keep) d.FrcInsX[o] = fix;
   R m9r = (R)d.MASK9[o];
   R hmax = rmax(d.H[o], mdf);
   R divpq = dv(d.P[IX(i + 1, j)] - d.P[o], dx) + dv(d.Q[IX(i, j + 1)] - d.Q[o], dy);
@@ -916,7 +918,7 @@ __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
          + g1 * m9r * divpq * (U4[o] - d.U1p[o]) + wm * u;
   if (p.FRIC_SPONGE) sx = sx - d.CD_4_SPONGE[o] * u * spd * d.Depth[o];
   R fiy = -(d.Cd[o] * v * spd);
-  d.FrcInsY[o] = fiy;
+  if (keep) d.FrcInsY[o] = fiy;
   R sy = dv(GRAV * (d.Eta[o]), dy) * (d.DepthY[IX(i, j + 1)] - d.DepthY[o]) * mk + fiy
          + g1 * m9r * hmax * (dv(u * (R)0.5 * (V4[IX(i + 1, j)] - V4[IX(i - 1, j)]), dx)
                               + dv(v * (R)0.5 * (V4[IX(i, j + 1)] - V4[IX(i, j - 1)]), dy)
@@ -939,23 +941,39 @@ __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
     const R h5x = p.h5x, h5y = p.h5y;
     R bx = h5x * (dv(sxp * (R)1, dx) * (HU[IX(i + 1, j)] - HU[o]) - dv(sxm * (R)1, dx) * (HU[o] - HU[IX(i - 1, j)]))
            + h5y * (dv(syp * (R)1, dy) * (HU[IX(i, j + 1)] - HU[o]) - dv(sym * (R)1, dy) * (HU[o] - HU[IX(i, j - 1)]));
-    d.BreakSourceX[o] = bx;
+    if (keep) d.BreakSourceX[o] = bx;
     sx = sx + bx;
     R by = h5x * (dv(sxp * (R)1, dx) * (HV[IX(i + 1, j)] - HV[o]) - dv(sxm * (R)1, dx) * (HV[o] - HV[IX(i - 1, j)]))
            + h5y * (dv(syp * (R)1, dy) * (HV[IX(i, j + 1)] - HV[o]) - dv(sym * (R)1, dy) * (HV[o] - HV[IX(i, j - 1)]));
-    d.BreakSourceY[o] = by;
+    if (keep) d.BreakSourceY[o] = by;
     sy = sy + by;
   }
-  d.SourceX[o] = sx;
-  d.SourceY[o] = sy;
+  if (keep) {
+    d.SourceX[o] = sx;
+    d.SourceY[o] = sy;
+  }
+  sx_out = sx;
+  sy_out = sy;
+  wm_out = wm;
+}
+template <class R>
+__global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
+  int i = g.Ib + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = g.Jb + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.Ie || j > g.Je) return;
+  R sx, sy, wm;
+  source_terms(d, g, p, AA, i, j, true, sx, sy, wm);
 }
 
 // --------------------------------------------------------------------------
 // ESTIMATE_HUV (etauv_solver.F:53-351): SSP-RK3 update of Eta, Ubar, Vbar;
 // then H = Eta*Gamma3 + Depth on all cells (GET_Eta_U_V_HU_HV :373) and the
 // normalised tridiagonal rows of both line systems (:400-539).
-template <class R>
-__global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT) {
+// FUSED: SourceTerms of the cell are formed here (source_terms) instead of
+// read back, which is the same arithmetic on the same inputs: SourceTerms
+// reads nothing ESTIMATE_HUV writes.
+template <class R, bool FUSED>
+__global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA) {
   int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
@@ -964,6 +982,14 @@ __global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT) {
   const R mdf = p.MDF;
   R eta = d.Eta[o];
   if (inner) {
+    R srcx, srcy, wmm;
+    if (FUSED) {
+      source_terms(d, g, p, AA, i, j, p.full_state != 0, srcx, srcy, wmm);
+    } else {
+      srcx = d.SourceX[o];
+      srcy = d.SourceY[o];
+      wmm = d.WaveMaker_Mass[o];
+    }
     const R *P = d.P, *Q = d.Q;
     R hcur = d.H[o];
     R pc = (R)0.5 * (P[IX(i + 1, j)] + P[o]);
@@ -977,7 +1003,7 @@ __global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT) {
     R r1 = -(p.rDXg * (fr * nr + fl * nl)) - p.rDYg * (ft * nr + fb * nl);
     if (p.wavemaker != 0) {
       R xmk = (R)(i - g.Ib) * p.DXg, ymk = (R)(j - g.Jb) * p.DYg;
-      if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < half(p.YW)) r1 = r1 + d.WaveMaker_Mass[o];
+      if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < half(p.YW)) r1 = r1 + wmm;
     }
     d.Wsurf[o] = r1;
     eta = al * d.Eta0[o] + be * (eta + DT * r1);
@@ -987,11 +1013,11 @@ __global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT) {
     }
     d.Eta[o] = eta;
     fl = d.Fx[o]; fr = d.Fx[IX(i + 1, j)]; fb = d.Fy[o]; ft = d.Fy[IX(i, j + 1)];
-    R r2 = -(p.rDX * (fr * nr + fl * nl)) - p.rDY * (ft * nr + fb * nl) + d.SourceX[o];
+    R r2 = -(p.rDX * (fr * nr + fl * nl)) - p.rDY * (ft * nr + fb * nl) + srcx;
     R ub = al * d.Ubar0[o] + be * (d.Ubar[o] + DT * r2);
     d.Ubar[o] = ub;
     fl = d.Gx[o]; fr = d.Gx[IX(i + 1, j)]; fb = d.Gy[o]; ft = d.Gy[IX(i, j + 1)];
-    R r3 = -(p.rDX * (fr * nr + fl * nl)) - p.rDY * (ft * nr + fb * nl) + d.SourceY[o];
+    R r3 = -(p.rDX * (fr * nr + fl * nl)) - p.rDY * (ft * nr + fb * nl) + srcy;
     R vb = al * d.Vbar0[o] + be * (d.Vbar[o] + DT * r3);
     d.Vbar[o] = vb;
     R h = eta * p.G3 + d.Depth[o];

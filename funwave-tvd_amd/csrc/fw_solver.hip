@@ -438,6 +438,9 @@ int do_sources(fw_ctx *c) {
 }
 
 template <class R>
+int do_solve(fw_ctx *c);
+
+template <class R>
 int do_huv(fw_ctx *c, int istage) {
   if (istage < 1 || istage > 3) return fail(c, FW_E_INVALID, "istage must be 1..3");
   // SSP-RK3 weights, mod_global.F:122-123
@@ -445,8 +448,33 @@ int do_huv(fw_ctx *c, int istage) {
   const R beta[3] = {(R)1, (R)1 / (R)4, (R)2 / (R)3};
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
-  launch(c, "huv", k_huv<R>, grid2(g.M, g.N, kB2), kB2, d, g, c->prm<R>(), alpha[istage - 1], beta[istage - 1],
-         (R)c->dt);
+  launch(c, "huv", k_huv<R, false>, grid2(g.M, g.N, kB2), kB2, d, g, c->prm<R>(), alpha[istage - 1],
+         beta[istage - 1], (R)c->dt, (R)0);
+  return do_solve<R>(c);
+}
+
+// SourceTerms fused into ESTIMATE_HUV (the whole-step path)
+template <class R>
+int do_src_huv(fw_ctx *c, int istage) {
+  if (istage < 1 || istage > 3) return fail(c, FW_E_INVALID, "istage must be 1..3");
+  const R alpha[3] = {(R)0, (R)3 / (R)4, (R)1 / (R)3};
+  const R beta[3] = {(R)1, (R)1 / (R)4, (R)2 / (R)3};
+  R AA;
+  int rc = stage_wavemaker<R>(c, AA);
+  if (rc) return rc;
+  Dev<R> &d = c->dev<R>();
+  const Geo &g = c->g;
+  launch(c, "src_huv", k_huv<R, true>, grid2(g.M, g.N, kB2), kB2, d, g, c->prm<R>(), alpha[istage - 1],
+         beta[istage - 1], (R)c->dt, AA);
+  return do_solve<R>(c);
+}
+
+// line solves and the final velocities (GET_Eta_U_V_HU_HV after the
+// coefficient assembly)
+template <class R>
+int do_solve(fw_ctx *c) {
+  Dev<R> &d = c->dev<R>();
+  const Geo &g = c->g;
   if (c->cfg.DISPERSION) {
     int ny = g.Je - g.Jb + 1, nx = g.Ie - g.Ib + 1;
     if (c->cfg.PERIODIC) {  // short periodic y lines: Sherman-Morrison per thread
@@ -501,8 +529,7 @@ int do_step(fw_ctx *c, int nsteps) {
     for (int st = 1; st <= 3; st++) {
       if (f.DISPERSION && (rc = do_dispersion<R>(c))) return rc;
       if ((rc = do_fluxes<R>(c))) return rc;
-      if ((rc = do_sources<R>(c))) return rc;
-      if ((rc = do_huv<R>(c, st))) return rc;
+      if ((rc = do_src_huv<R>(c, st))) return rc;
       if ((rc = do_mask<R>(c))) return rc;
       if ((rc = do_breaking<R>(c))) return rc;
       if ((rc = do_exchange<R>(c, f.DIRECT_SPONGE != 0))) return rc;
