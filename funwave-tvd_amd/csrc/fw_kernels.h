@@ -994,10 +994,12 @@ __global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA) {
     R hcur = d.H[o];
     R pc = (R)0.5 * (P[IX(i + 1, j)] + P[o]);
     R qc = (R)0.5 * (Q[IX(i, j + 1)] + Q[o]);
-    d.P_center[o] = pc;
-    d.Q_center[o] = qc;
-    d.U_davg[o] = ddiv(pc, rmax(hcur, mdf));
-    d.V_davg[o] = ddiv(qc, rmax(hcur, mdf));
+    if (p.full_state) {  // mean-flow diagnostics (etauv_solver.F:96-99, :247)
+      d.P_center[o] = pc;
+      d.Q_center[o] = qc;
+      d.U_davg[o] = ddiv(pc, rmax(hcur, mdf));
+      d.V_davg[o] = ddiv(qc, rmax(hcur, mdf));
+    }
     const R nl = (R)-1, nr = (R)1;
     R fl = P[o], fr = P[IX(i + 1, j)], fb = Q[o], ft = Q[IX(i, j + 1)];
     R r1 = -(p.rDXg * (fr * nr + fl * nl)) - p.rDYg * (ft * nr + fb * nl);
@@ -1005,7 +1007,7 @@ __global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA) {
       R xmk = (R)(i - g.Ib) * p.DXg, ymk = (R)(j - g.Jb) * p.DYg;
       if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < half(p.YW)) r1 = r1 + wmm;
     }
-    d.Wsurf[o] = r1;
+    if (p.full_state) d.Wsurf[o] = r1;
     eta = al * d.Eta0[o] + be * (eta + DT * r1);
     if (p.ETA_LIMITER) {
       if (eta < p.TROUGH) eta = p.TROUGH;

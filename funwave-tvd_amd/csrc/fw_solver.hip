@@ -135,6 +135,9 @@ int collect_profile(fw_ctx *c) {
 
 inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
 const dim3 kB2(64, 4);
+// taller tiles for kernels reading j-1/j+1 neighbours: fewer rows fetched
+// twice by vertically adjacent workgroups
+const dim3 kBT(64, 8);
 
 template <class R>
 void fill_prm(fw_ctx *c) {
@@ -330,11 +333,13 @@ int do_exchange(fw_ctx *c, bool sponge) {
   const fw_config &f = c->cfg;
   HaloList L{}, Lf{};
   hadd(L, d.Eta, 1);
-  hadd(L, d.Wsurf, 1);
-  hadd(L, d.P_center, 1);
-  hadd(L, d.Q_center, 1);
-  hadd(L, d.U_davg, 1);
-  hadd(L, d.V_davg, 1);
+  if (c->prm<R>().full_state) {  // mean-flow diagnostics, read by no routine of the step
+    hadd(L, d.Wsurf, 1);
+    hadd(L, d.P_center, 1);
+    hadd(L, d.Q_center, 1);
+    hadd(L, d.U_davg, 1);
+    hadd(L, d.V_davg, 1);
+  }
   if (f.VISCOSITY_BREAKING) hadd(L, d.AGE_BREAKING, 1);
   if (f.VISCOSITY_BREAKING) hadd(L, d.nu_break, 1);
   if (f.VISCOSITY_BREAKING || f.DIFFUSION_SPONGE) {
@@ -391,7 +396,7 @@ template <class R>
 int do_dispersion(fw_ctx *c) {
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
-  launch(c, "disp_deriv", k_disp_deriv<R>, grid2(g.M, g.N, kB2), kB2, d, g, c->prm<R>(), (R)c->dt);
+  launch(c, "disp_deriv", k_disp_deriv<R>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), (R)c->dt);
   HaloList L{}, Lf{};
   hadd(L, d.Uxx, 2); hadd(L, d.DUxx, 2); hadd(L, d.Vyy, 3); hadd(L, d.DVyy, 3);
   hadd(L, d.Uxy, 1); hadd(L, d.DUxy, 1); hadd(L, d.Vxy, 1); hadd(L, d.DVxy, 1);
@@ -406,7 +411,7 @@ int do_dispersion(fw_ctx *c) {
     hadd(L, d.ETAx, 2); hadd(L, d.ETAy, 3);
   }
   halo_exchange_list<R>(c, L, Lf);
-  launch(c, "disp_terms", k_disp_terms<R>, grid2(g.M, g.N, kB2), kB2, d, g, c->prm<R>(), (R)c->dt);
+  launch(c, "disp_terms", k_disp_terms<R>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), (R)c->dt);
   return FW_OK;
 }
 
@@ -448,7 +453,7 @@ int do_huv(fw_ctx *c, int istage) {
   const R beta[3] = {(R)1, (R)1 / (R)4, (R)2 / (R)3};
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
-  launch(c, "huv", k_huv<R, false>, grid2(g.M, g.N, kB2), kB2, d, g, c->prm<R>(), alpha[istage - 1],
+  launch(c, "huv", k_huv<R, false>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), alpha[istage - 1],
          beta[istage - 1], (R)c->dt, (R)0);
   return do_solve<R>(c);
 }
@@ -464,7 +469,7 @@ int do_src_huv(fw_ctx *c, int istage) {
   if (rc) return rc;
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
-  launch(c, "src_huv", k_huv<R, true>, grid2(g.M, g.N, kB2), kB2, d, g, c->prm<R>(), alpha[istage - 1],
+  launch(c, "src_huv", k_huv<R, true>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), alpha[istage - 1],
          beta[istage - 1], (R)c->dt, AA);
   return do_solve<R>(c);
 }
@@ -499,7 +504,7 @@ template <class R>
 int do_mask(fw_ctx *c) {
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
-  launch(c, "mask", k_mask<R>, grid2(g.M + 1, g.N + 1, kB2), kB2, d, g, c->prm<R>());
+  launch(c, "mask", k_mask<R>, grid2(g.M + 1, g.N + 1, kBT), kBT, d, g, c->prm<R>());
   std::swap(d.MASK, d.MASKn);
   return FW_OK;
 }
@@ -509,7 +514,7 @@ int do_breaking(fw_ctx *c) {
   if (!c->cfg.SHOW_BREAKING) return FW_OK;
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
-  launch(c, "brk1", k_brk1<R>, grid2(g.M, g.N, kB2), kB2, d, g, c->prm<R>(), (R)c->dt);
+  launch(c, "brk1", k_brk1<R>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), (R)c->dt);
   launch(c, "brk2", k_brk2<R>, dim3(1), dim3(1024), d, g, c->prm<R>(), (R)c->dt);
   std::swap(d.AGE_BREAKING, d.AGEn);
   return FW_OK;

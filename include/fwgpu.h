@@ -119,11 +119,12 @@ int fw_reduce_stats(fw_ctx *ctx, fw_stats *out);
  * date (face states UxL..GyR, HxL.., SxL.., the wave-speed copies, the
  * first derivatives Ut, Vt, Ux.., Utx.., DUtxx.. of CAL_DISPERSION, and in
  * fw_step the source terms SourceX/Y, FrcInsX/Y, BreakSourceX/Y and
- * WaveMaker_Mass), for routine-by-routine comparison with the reference.
+ * WaveMaker_Mass, and the mean-flow diagnostics P_center, Q_center, U_davg,
+ * V_davg, Wsurf), for routine-by-routine comparison with the reference.
  * Default 0: only the arrays a later routine reads are written; results are
- * identical.  fw_source_terms always writes its outputs; the face states
- * and the dispersion first derivatives follow this option in the
- * per-routine entry points too. */
+ * identical.  fw_source_terms always writes its outputs; the face states,
+ * the dispersion first derivatives and the mean-flow diagnostics follow
+ * this option in the per-routine entry points too. */
 enum { FW_OPT_FULL_STATE = 1 };
 int fw_set_option(fw_ctx *ctx, int option, int value);
 

@@ -36,7 +36,9 @@ DISP_FIRST = {"Ut", "Vt", "Ux", "Vx", "Uy", "Vy", "DUx", "DVx", "DUy", "DVy", "U
               "Utxx", "Utxy", "Vtxy", "Vtyy", "DUtx", "DVty", "DUtxx", "DUtxy", "DVtxy", "DVtyy"}
 # and the source terms, formed inside the RK update of their own cell
 SOURCES = {"SourceX", "SourceY", "WaveMaker_Mass", "BreakSourceX", "BreakSourceY", "FrcInsX", "FrcInsY"}
-PRODUCTION_SKIP = FACE_STATES | DISP_FIRST | SOURCES
+# and the mean-flow diagnostics of ESTIMATE_HUV, read by no routine of the step
+DIAGNOSTICS = {"P_center", "Q_center", "U_davg", "V_davg", "Wsurf"}
+PRODUCTION_SKIP = FACE_STATES | DISP_FIRST | SOURCES | DIAGNOSTICS
 
 
 def _compare(g, o, where, full_state=True):
