@@ -691,6 +691,25 @@ int fw_device_bytes(const fw_ctx *c, int64_t *bytes) {
   return FW_OK;
 }
 
+int fw_field_device(fw_ctx *c, int field, void **ptr, int *pitch) {
+  if (!c || !ptr || field < 0 || field >= F_xmk_wk) return fail(c, FW_E_INVALID, "field has no device plane");
+  *ptr = field_ptr(c, field);
+  if (pitch) *pitch = c->g.P;
+  return FW_OK;
+}
+
+int fw_get_stream(fw_ctx *c, void **stream) {
+  if (!c || !stream) return FW_E_INVALID;
+  *stream = (void *)c->stream;
+  return FW_OK;
+}
+
+int fw_get_config(const fw_ctx *c, fw_config *cfg) {
+  if (!c || !cfg) return FW_E_INVALID;
+  *cfg = c->cfg;
+  return FW_OK;
+}
+
 int fw_field_id(const char *name) {
   if (!name) return FW_E_NOTFOUND;
   for (int k = 0; k < F_COUNT; k++)

@@ -40,7 +40,8 @@ EXPORTS = ["fw_create", "fw_destroy", "fw_strerror", "fw_synchronize",
            "fw_estimate_huv", "fw_update_mask", "fw_wave_breaking",
            "fw_sponge_damping", "fw_check_blowup", "fw_step", "fw_reduce_stats",
            "fw_set_option", "fw_profile_enable", "fw_profile_reset", "fw_profile_get",
-           "fw_profile_names", "fw_time_steps"]
+           "fw_profile_names", "fw_time_steps", "fw_field_device", "fw_get_stream",
+           "fw_get_config"]
 
 
 class Config(ct.Structure):

@@ -90,6 +90,16 @@ int fw_download(fw_ctx *ctx, int field, void *host);
 int fw_set_clock(fw_ctx *ctx, double dt, double time);
 int fw_get_clock(const fw_ctx *ctx, double *dt, double *time);
 
+/* --- device access --------------------------------------------------------
+ * For libraries that work on the HBM state in place, on the context's
+ * stream: the device address of a field's plane (element (I,J) at
+ * ptr[(J-1)*pitch + (I-1)], pitch in elements; masks are int32, reals
+ * REAL(SP)), the stream itself (a hipStream_t), and the configuration the
+ * context was created with. */
+int fw_field_device(fw_ctx *ctx, int field, void **ptr, int *pitch);
+int fw_get_stream(fw_ctx *ctx, void **stream);
+int fw_get_config(const fw_ctx *ctx, fw_config *cfg);
+
 /* --- per-step entry points (same order and meaning as src/main.F) ------- */
 int fw_exchange(fw_ctx *ctx);                /* EXCHANGE      bc.F:426      */
 int fw_estimate_dt(fw_ctx *ctx, double *dt, double *time);

@@ -1,5 +1,5 @@
-"""The C-ABI library: it loads and exports every function include/fwgpu.h
-declares; host-only calls work without a GPU."""
+"""The C-ABI library: it loads and exports every function include/*.h
+declare; host-only calls work without a GPU."""
 import ctypes as ct
 import os
 import re
@@ -11,8 +11,8 @@ from common import REPO
 LIB = os.path.join(REPO, "funwave-tvd_amd", "libfwgpu.so")
 
 
-def declared():
-    src = open(os.path.join(REPO, "include", "fwgpu.h")).read()
+def declared(header="fwgpu.h"):
+    src = open(os.path.join(REPO, "include", header)).read()
     return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(fw_\w+)\s*\(", src, re.M)))
 
 
@@ -25,6 +25,13 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     import fwgpu
     assert set(fwgpu.EXPORTS) == set(names)
+    cnn = declared("fwcnn.h")
+    assert len(cnn) >= 8
+    missing = [n for n in cnn if not hasattr(lib, n)]
+    assert not missing, missing
+    import fwcnn
+    assert set(fwcnn.EXPORTS) == set(cnn)
+    assert sorted(os.listdir(os.path.join(REPO, "include"))) == ["fwcnn.h", "fwgpu.h"]
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libfwgpu.so not built")
