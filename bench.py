@@ -41,6 +41,11 @@ KERNEL_TRAFFIC = {
                                          #   -> U4 V4 U1p V1p U1pp V1pp U2 V2 U3 V3
     "sources": (24, 2, 2, "inner"),
     "huv": (24, 2, 12, "cell"),
+    # SourceTerms fused into ESTIMATE_HUV: U V U4 V4 Cd H P Q Eta DepthX DepthY
+    # U1p V1p U1pp V1pp U2 V2 U3 V3 nu_break HU HV Eta0 Ubar0 Vbar0 Ubar Vbar
+    # Fx Fy Gx Gy Depth Uxy DUxy Vxy DVxy, MASK MASK9
+    #   -> Eta Ubar Vbar H and the six line-solve coefficient arrays
+    "src_huv": (36, 2, 10, "cell"),
     "trid": (6, 0, 2, "inner"),
     "uv_final": (3, 1, 4, "cell"),
     "mask": (4, 2, 2, "cell"),
@@ -60,7 +65,56 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=512)
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--ensemble", action="store_true",
+                    help="measure learned coarse vs plain fine ensemble runs/hour instead")
+    ap.add_argument("--train-members", type=int, default=8)
+    ap.add_argument("--eval-members", type=int, default=8)
+    ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--fine-steps", type=int, default=200)
     return ap.parse_args()
+
+
+def ensemble(args):
+    """Learned coarse vs plain fine ensemble (conical-island INI_SOL members,
+    Mglob ~ U[256,768], Nglob ~ U[128,512], dx = 4 m; the learned runs use
+    dx = 8 m and the network correction after every step).  Prints one JSON
+    line; the learned rate counts only if every member's final eta is within
+    2 % relative L2 of the 2x2-averaged fine solution."""
+    import numpy as np
+    import fwlearn
+    t0 = time.time()
+    members = []
+    for seed in range(5000, 5000 + args.train_members):
+        c = fwlearn.member(seed)
+        members.append(fwlearn.with_fixed_dt(c, fwlearn.stable_dt(c)))
+    data = [fwlearn.make_samples(c, n_samples=16, every=6) for c in members]
+    t_data = time.time() - t0
+    params, losses = fwlearn.train(data, steps=args.train_steps, batch=16, size=64, lr=2e-3)
+    t_train = time.time() - t0 - t_data
+    evals = []
+    for seed in range(6000, 6000 + args.eval_members):
+        c = fwlearn.member(seed)
+        evals.append(fwlearn.with_fixed_dt(c, fwlearn.stable_dt(c)))
+    r = fwlearn.compare(evals, args.fine_steps, params)
+    rph = r["runs_per_hour"]
+    ok = r["all_learned_within_bound"]
+    out = {
+        "metric": "ensemble runs/hr learned vs plain at eta err<=2%",
+        "value": round(rph["learned"], 1) if ok else None, "unit": "runs/hour", "n_gpus": 1,
+        "higher_is_better": True, "data": "synthetic", "dtype": "f32 solver, bf16 network",
+        "accepted": ok, "plain_fine_runs_per_hour": round(rph["fine"], 1),
+        "learned_runs_per_hour": round(rph["learned"], 1), "plain_coarse_runs_per_hour": round(rph["coarse"], 1),
+        "speedup_learned_vs_fine": round(rph["learned"] / rph["fine"], 3),
+        "max_err_learned": round(r["max_err_learned"], 4), "max_err_coarse": round(r["max_err_coarse"], 4),
+        "config": {"workload": "conical-island solitary-wave members (INI_SOL), fixed dt, "
+                               f"{args.fine_steps} fine steps each, members run one after another",
+                   "train_members": args.train_members, "eval_members": args.eval_members,
+                   "train_steps": args.train_steps, "loss_first": float(np.mean(losses[:20])),
+                   "loss_last": float(np.mean(losses[-20:])),
+                   "data_s": round(t_data, 1), "train_s": round(t_train, 1)},
+        "members": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in m.items()} for m in r["members"]],
+    }
+    print(json.dumps(out))
 
 
 def sync():
@@ -115,6 +169,9 @@ def main():
         print(f"bench.py: --gpus {args.gpus} is not supported; this build runs one domain on one GPU "
               "(multi-GPU sharding is design-only).", file=sys.stderr)
         sys.exit(2)
+    if args.ensemble:
+        ensemble(args)
+        return
     import numpy as np  # noqa: F401
     import fwcases
     import fwgpu

@@ -161,7 +161,9 @@ def build_state(case, precision: str = "sp") -> dict:
               DIFFUSION_SPONGE=int(bool(p.get("DIFFUSION_SPONGE", False))),
               FRICTION_SPONGE=int(bool(p.get("FRICTION_SPONGE", False))),
               DIRECT_SPONGE=int(bool(p.get("DIRECT_SPONGE", False))),
-              ETA_LIMITER=0, FIXED_DT=0, DT_fixed=0.0, CrestLimit=0.0, TroughLimit=0.0,
+              # a DT_fixed entry switches FIXED_DT on (io.F:2882-2896)
+              ETA_LIMITER=0, FIXED_DT=int("DT_fixed" in p), DT_fixed=float(p.get("DT_fixed", 0.0)),
+              CrestLimit=0.0, TroughLimit=0.0,
               T_brk=20.0, WaveMaker=str(p.get("WAVEMAKER", "NONE")))
     if st["FRICTION_SPONGE"] or st["DIFFUSION_SPONGE"]:
         raise NotImplementedError("friction / diffusion sponge set-up is not restated here")
@@ -276,7 +278,8 @@ def build_state(case, precision: str = "sp") -> dict:
     if wk.startswith("INI_SOL"):
         _, bue, ae1, ae2, au = _solitary_coeffs(float(p["AMP"]), float(p["DEP"]), rdt)
         I = np.arange(1, M + 1).astype(rdt)
-        sc = rdt(1) / np.cosh(bue * (I - rdt(float(p["XWAVEMAKER"])) / rdt(dx) - rdt(1)) * rdt(dx))
+        with np.errstate(over="ignore"):  # far from the crest cosh overflows: sech = 0
+            sc = rdt(1) / np.cosh(bue * (I - rdt(float(p["XWAVEMAKER"])) / rdt(dx) - rdt(1)) * rdt(dx))
         e1 = ae1 * sc * sc + ae2 * sc * sc * sc * sc
         u1 = au * sc * sc
         if not bool(p.get("SolitaryPositiveDirection", True)):
