@@ -125,6 +125,29 @@ def sync():
         torch.cuda.synchronize()
 
 
+# rocprofv3 names of the kernels behind each profiling name
+ROCPROF_NAME = {"trid": "k_trid_lds<", "src_huv": "k_huv<{r}, true>", "huv": "k_huv<{r}, false>",
+                "flux_x": "k_flux_x<", "flux_y": "k_flux_y<", "disp_deriv": "k_disp_deriv<",
+                "disp_terms": "k_disp_terms<", "sources": "k_sources<", "uv_final": "k_uv_final<",
+                "mask": "k_mask<", "brk1": "k_brk1<", "mask_mul": "k_mask_mul<"}
+
+
+def measured_traffic(kernel, rb, size):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, calibrated on a
+    copy kernel), or None if there is none for this size / precision."""
+    tag = f"beach{size}_{'fp64' if rb == 8 else 'fp32'}"
+    path = os.path.join(REPO, "profiles", f"counters_{tag}.md")
+    if kernel not in ROCPROF_NAME or not os.path.exists(path):
+        return None, None
+    want = ROCPROF_NAME[kernel].format(r="double" if rb == 8 else "float")
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0].startswith(want) and (("double" if rb == 8 else "float") in cells[0]):
+            return float(cells[4]) * 1e6, os.path.relpath(path, REPO)
+    return None, None
+
+
 def kernel_traffic_bytes(name, rb, M, N, mg, ng):
     r, i, w, unit = KERNEL_TRAFFIC[name]
     units = {"cell": M * N, "xface": (M + 1) * N, "yface": M * (N + 1), "inner": mg * ng}[unit]
@@ -212,6 +235,7 @@ def main():
     avg_s = ms / cnt / 1e3
     alg = kernel_traffic_bytes(dom, rb, M, N, n, n)
     achieved = alg / avg_s
+    traffic, traffic_src = measured_traffic(dom, rb, n)
     step_s = wall / args.steps
     step_alg = B_ALG_WORDS * rb * n * n
     out = {
@@ -226,8 +250,8 @@ def main():
                    "parallelism": "one domain on one GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
-                     "alg_bytes_per_launch": alg},
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": alg},
         "step_roofline": {"alg_bytes_per_cell_step": B_ALG_WORDS * rb,
                           "achieved_GBps": round(step_alg / step_s / 1e9, 2),
                           "frac": round(step_alg / step_s / HBM_PEAK, 4),
