@@ -26,6 +26,7 @@ sys.path.insert(0, os.path.join(REPO, "funwave-tvd_amd"))
 
 METRIC = "Mcell-steps/s at 2048^2; ensemble runs/hr learned vs plain at eta err<=2%"
 HBM_PEAK = 8.0e12           # MI355X HBM3E, B/s
+BF16_PEAK = 2.5e15          # MI355X dense bf16 MFMA, FLOP/s
 B_ALG_WORDS = 93            # algorithmic words per interior cell-step (whole step)
 
 # Algorithmic traffic of each kernel per unit of its launch: (real words
@@ -148,6 +149,23 @@ def measured_traffic(kernel, rb, size):
     return None, None
 
 
+def cnn_roofline(n=1024, iters=20):
+    """The learned correction's network (four bf16 MFMA convolutions) on the
+    1024 x 1024 coarse grid of the 2048^2 workload: device time per forward
+    pass and its algorithmic FLOP rate against the dense bf16 peak."""
+    import numpy as np
+    import fwcnn
+    net = fwcnn.Net(n, n, 1, fwcnn.init_params(0))
+    net.forward(np.zeros((1, 5, n, n), np.float32))
+    ms = net.time_forward(1, iters)
+    net.close()
+    flops = fwcnn.FLOP_PER_CELL * n * n
+    return {"kernel": "k_conv x4 (bf16 MFMA 16x16x32, fp32 accumulate)", "cells": n * n,
+            "ms_per_forward": round(ms, 4), "achieved": round(flops / (ms * 1e-3) / 1e12, 2),
+            "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s", "frac": round(flops / (ms * 1e-3) / BF16_PEAK, 4),
+            "flop_per_cell": fwcnn.FLOP_PER_CELL}
+
+
 def kernel_traffic_bytes(name, rb, M, N, mg, ng):
     r, i, w, unit = KERNEL_TRAFFIC[name]
     units = {"cell": M * N, "xface": (M + 1) * N, "yface": M * (N + 1), "inner": mg * ng}[unit]
@@ -260,6 +278,7 @@ def main():
                                 sorted(kern.items(), key=lambda kv: -kv[1][1])},
         "setup_s": round(t_setup, 1),
     }
+    out["learned_correction"] = cnn_roofline()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args, args.precision)
     g.close()

@@ -571,6 +571,26 @@ int fw_cnn_train_step(fw_cnn *n, const float *in, const float *target, int B, fl
   return FW_OK;
 }
 
+int fw_cnn_time_forward(fw_cnn *n, int B, int iters, double *ms) {
+  int rc = check_batch(n, B);
+  if (rc) return rc;
+  if (iters < 1 || !ms) return cfail(n, FW_E_INVALID, "iters must be >= 1");
+  hipEvent_t a, b;
+  CHK(n, hipEventCreate(&a));
+  CHK(n, hipEventCreate(&b));
+  forward(n, B);  // warm-up
+  CHK(n, hipEventRecord(a, n->stream));
+  for (int k = 0; k < iters; k++) forward(n, B);
+  CHK(n, hipEventRecord(b, n->stream));
+  CHK(n, hipEventSynchronize(b));
+  float t = 0.f;
+  CHK(n, hipEventElapsedTime(&t, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = (double)t / iters;
+  return FW_OK;
+}
+
 int fw_cnn_correct(fw_cnn *n, fw_ctx *solver, const float fs[4], const float os[3]) {
   if (!n || !solver || !fs || !os) return FW_E_INVALID;
   fw_config cfg;

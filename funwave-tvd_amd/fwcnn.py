@@ -18,7 +18,10 @@ NPARAMS = 9 * 5 * 32 + 32 + 2 * (9 * 32 * 32 + 32) + 9 * 32 * 3 + 3
 SHAPES = [("W1", (9, 5, 32)), ("b1", (32,)), ("W2", (9, 32, 32)), ("b2", (32,)),
           ("W3", (9, 32, 32)), ("b3", (32,)), ("W4", (9, 32, 3)), ("b4", (3,))]
 EXPORTS = ["fw_cnn_create", "fw_cnn_destroy", "fw_cnn_strerror", "fw_cnn_set_params",
-           "fw_cnn_get_params", "fw_cnn_forward", "fw_cnn_train_step", "fw_cnn_correct"]
+           "fw_cnn_get_params", "fw_cnn_forward", "fw_cnn_train_step", "fw_cnn_correct",
+           "fw_cnn_time_forward"]
+# multiply-adds x 2 per cell of the four layers (algorithmic, unpadded channels)
+FLOP_PER_CELL = 2 * 9 * (5 * 32 + 32 * 32 + 32 * 32 + 32 * 3)
 # feature scales: eta / 1 m, u, v / 1 m/s, depth / 10 m, mask
 FEATURE_SCALE = np.array([1.0, 1.0, 1.0, 0.1], np.float32)
 # outputs are corrections in units of 1 mm (eta) and 1 mm/s (u, v)
@@ -67,6 +70,7 @@ def _lib():
         lib.fw_cnn_forward.argtypes = [vp, fp, fp, ip]
         lib.fw_cnn_train_step.argtypes = [vp, fp, fp, ip, ct.c_float, fp, fp]
         lib.fw_cnn_correct.argtypes = [vp, vp, fp, fp]
+        lib.fw_cnn_time_forward.argtypes = [vp, ip, ip, ct.POINTER(ct.c_double)]
         lib._fwcnn_typed = True
     return lib
 
@@ -130,6 +134,12 @@ class Net:
         self._check(self.lib.fw_cnn_train_step(self.ptr, _fptr(x), _fptr(t), B, lr, ct.byref(loss),
                                                _fptr(g) if want_grad else None))
         return float(loss.value), g
+
+    def time_forward(self, B: int = 1, iters: int = 20) -> float:
+        """Device milliseconds per forward pass over B resident samples."""
+        ms = ct.c_double()
+        self._check(self.lib.fw_cnn_time_forward(self.ptr, B, iters, ct.byref(ms)))
+        return ms.value
 
     def correct(self, solver: "fwgpu.Solver", fs=FEATURE_SCALE, os_=OUTPUT_SCALE):
         """Queue the in-place correction of a coarse solver's state."""

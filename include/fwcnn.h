@@ -51,6 +51,11 @@ int fw_cnn_forward(fw_cnn *net, const float *in, float *out, int B);
 int fw_cnn_train_step(fw_cnn *net, const float *in, const float *target, int B, float lr, float *loss,
                       float *grad);
 
+/* Device milliseconds per forward pass over B resident samples (the
+ * current input buffer), averaged over iters passes timed with HIP events
+ * on the network's stream. */
+int fw_cnn_time_forward(fw_cnn *net, int B, int iters, double *ms);
+
 /* In-place correction of a coarse solver state whose interior is H x W:
  * features (eta*fs[0], u*fs[1], v*fs[2], depth*fs[3], mask) of the current
  * state, network output o, then eta += o0*os[0], u += o1*os[1],
