@@ -792,80 +792,112 @@ __device__ __forceinline__ void yface_finish(const Dev<R> &d, const Geo &g, cons
 }
 
 // y faces: the same cell-wise reconstruction as along x, with the
-// neighbours exchanged through LDS.  A workgroup of 64 x kFluxYRows threads
-// covers 64 columns and kFluxYRows consecutive faces j of each; thread row r
-// forms D(j) and the pair of cell j, takes D(j+1) from row r+1 and the left
-// state of face j from row r-1.  Rows 1..kFluxYRows-2 own their face, so
-// consecutive workgroups overlap by two rows.
-constexpr int kFluxYRows = 16;
-constexpr int kFluxYFaces = kFluxYRows - 2;
+// neighbours exchanged through LDS.  A workgroup of 64 x 16 threads covers 64
+// columns and 16*K consecutive faces j of each (thread row t handles tile
+// rows t, t+16, ...); tile row r forms D(j) and the pair of cell j, takes
+// D(j+1) from row r+1 and the left state of face j from row r-1.  Rows
+// 1..16K-2 own their face, so consecutive workgroups overlap by two rows.
+constexpr int kFluxYThreadRows = 16;
+template <int K>
+struct FluxYTile {
+  static constexpr int rows = kFluxYThreadRows * K;
+  static constexpr int faces = rows - 2;
+};
+// rows per thread: 2 in FP32, 1 in FP64 (LDS holds 7 fields x rows x 64)
+template <class R>
+struct FluxYK {
+  static constexpr int K = sizeof(R) == 4 ? 2 : 1;
+};
 
-template <class R, bool DISP>
-__global__ void __launch_bounds__(64 * kFluxYRows) k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
+template <class R, bool DISP, int K>
+__global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
   constexpr int NV = DISP ? 7 : 5;
-  // per field: D(j) of every thread row, then (after the pairs are formed)
-  // the left state of face j+1
-  __shared__ R sX[NV][kFluxYRows][64];
+  constexpr int ROWS = FluxYTile<K>::rows, FACES = FluxYTile<K>::faces;
+  // per field: D(j) of every tile row, then (after the pairs are formed) the
+  // left state of face j+1
+  __shared__ R sX[NV][ROWS][64];
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i = 1 + blockIdx.x * 64 + tx;
-  const int j = blockIdx.y * kFluxYFaces + ty;  // face index of this thread row
   const int N1 = g.N + 1;
   const ptrdiff_t s = (ptrdiff_t)g.P;
   const bool col = i >= g.Ib && i <= g.Ie;
-  const bool own = ty >= 1 && ty <= kFluxYFaces && j <= N1 && i <= g.M;
-  const bool comp = col && j >= g.Jb && j <= g.Je + 1;
-  // D(j) needs cells j-2..j+1, the pair of cell j needs D(j), D(j+1)
-  const bool dok = col && j >= g.Jb - 1 && j <= g.Je + 2;
-  const bool pok = col && j >= g.Jb - 1 && j <= g.Je + 1 && ty + 1 < kFluxYRows;
-  const size_t o = IX(i <= g.M ? i : g.M, j < 1 ? 1 : (j > N1 ? N1 : j));
   const R *fv[7] = {d.U, d.V, d.HU, d.HV, d.Eta, d.U4, d.V4};
-  R w[NV][4];
-  int mk[4] = {0, 0, 0, 0};
-  if (dok) {
+  R vc[K][NV], D[K][NV];
 #pragma unroll
-    for (int k = 0; k < 4; k++) mk[k] = d.MASK[o + (k - 2) * s];
+  for (int k = 0; k < K; k++) {
+    const int r = ty + kFluxYThreadRows * k;
+    const int j = blockIdx.y * FACES + r;
+    // D(j) needs cells j-2..j+1
+    const bool dok = col && j >= g.Jb - 1 && j <= g.Je + 2;
+    const size_t o = IX(i <= g.M ? i : g.M, j < 1 ? 1 : (j > N1 ? N1 : j));
+    int mk[4] = {0, 0, 0, 0};
+    R w[NV][4];
+    if (dok) {
 #pragma unroll
-    for (int q = 0; q < NV; q++)
+      for (int t = 0; t < 4; t++) mk[t] = d.MASK[o + (t - 2) * s];
 #pragma unroll
-      for (int k = 0; k < 4; k++) w[q][k] = fv[q][o + (k - 2) * s];
-  }
-  R D[NV];
+      for (int q = 0; q < NV; q++)
 #pragma unroll
-  for (int q = 0; q < NV; q++) {
-    D[q] = dok ? din_val(w[q][0], w[q][1], w[q][2], w[q][3], mk[0], mk[1], mk[2], mk[3]) : (R)0;
-    sX[q][ty][tx] = D[q];
-  }
-  __syncthreads();
-  R Dn[NV];
+        for (int t = 0; t < 4; t++) w[q][t] = fv[q][o + (t - 2) * s];
+    }
 #pragma unroll
-  for (int q = 0; q < NV; q++) Dn[q] = ty + 1 < kFluxYRows ? sX[q][ty + 1][tx] : (R)0;
-  __syncthreads();
-  R Rf[NV];
-#pragma unroll
-  for (int q = 0; q < NV; q++) {
-    R Ln = (R)0, Rc = (R)0;
-    if (pok) pair_states(w[q][2], D[q], Dn[q], Ln, Rc);
-    Rf[q] = Rc;
-    sX[q][ty][tx] = Ln;
+    for (int q = 0; q < NV; q++) {
+      D[k][q] = dok ? din_val(w[q][0], w[q][1], w[q][2], w[q][3], mk[0], mk[1], mk[2], mk[3]) : (R)0;
+      vc[k][q] = dok ? w[q][2] : (R)0;
+      sX[q][r][tx] = D[k][q];
+    }
   }
   __syncthreads();
-  if (!own) return;
-  R Lf[NV];
+  R Dn[K][NV];
 #pragma unroll
-  for (int q = 0; q < NV; q++) Lf[q] = sX[q][ty - 1][tx];
-  R uL = Lf[0], uR = Rf[0], vL = Lf[1], vR = Rf[1], huL = Lf[2], huR = Rf[2], hvL = Lf[3], hvR = Rf[3];
-  R eL = Lf[4], eR = Rf[4], u4L, u4R, v4L, v4R;
-  if (!comp) {
-    uL = d.UyL[o]; uR = d.UyR[o]; vL = d.VyL[o]; vR = d.VyR[o];
-    huL = d.HUyL[o]; huR = d.HUyR[o]; hvL = d.HVyL[o]; hvR = d.HVyR[o];
-    eL = d.EtaRyL[o]; eR = d.EtaRyR[o];
+  for (int k = 0; k < K; k++) {
+    const int r = ty + kFluxYThreadRows * k;
+#pragma unroll
+    for (int q = 0; q < NV; q++) Dn[k][q] = r + 1 < ROWS ? sX[q][r + 1][tx] : (R)0;
   }
-  if (DISP && comp) {
-    u4L = Lf[5 % NV]; u4R = Rf[5 % NV]; v4L = Lf[6 % NV]; v4R = Rf[6 % NV];
-  } else {
-    u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
+  __syncthreads();
+  R Rf[K][NV];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int r = ty + kFluxYThreadRows * k;
+    const int j = blockIdx.y * FACES + r;
+    // the pair of cell j needs D(j), D(j+1)
+    const bool pok = col && j >= g.Jb - 1 && j <= g.Je + 1 && r + 1 < ROWS;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      R Ln = (R)0, Rc = (R)0;
+      if (pok) pair_states(vc[k][q], D[k][q], Dn[k][q], Ln, Rc);
+      Rf[k][q] = Rc;
+      sX[q][r][tx] = Ln;
+    }
   }
-  yface_finish(d, g, p, i, j, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int r = ty + kFluxYThreadRows * k;
+    const int j = blockIdx.y * FACES + r;
+    const bool own = r >= 1 && r <= FACES && j <= N1 && i <= g.M;
+    if (!own) continue;
+    const bool comp = col && j >= g.Jb && j <= g.Je + 1;
+    const size_t o = IX(i, j);
+    R Lf[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) Lf[q] = sX[q][r - 1][tx];
+    R uL = Lf[0], uR = Rf[k][0], vL = Lf[1], vR = Rf[k][1], huL = Lf[2], huR = Rf[k][2], hvL = Lf[3],
+      hvR = Rf[k][3];
+    R eL = Lf[4], eR = Rf[k][4], u4L, u4R, v4L, v4R;
+    if (!comp) {
+      uL = d.UyL[o]; uR = d.UyR[o]; vL = d.VyL[o]; vR = d.VyR[o];
+      huL = d.HUyL[o]; huR = d.HUyR[o]; hvL = d.HVyL[o]; hvR = d.HVyR[o];
+      eL = d.EtaRyL[o]; eR = d.EtaRyR[o];
+    }
+    if (DISP && comp) {
+      u4L = Lf[5 % NV]; u4R = Rf[k][5 % NV]; v4L = Lf[6 % NV]; v4R = Rf[k][6 % NV];
+    } else {
+      u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
+    }
+    yface_finish(d, g, p, i, j, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
+  }
 }
 
 // --------------------------------------------------------------------------
