@@ -1588,6 +1588,9 @@ __global__ void k_mask_mul(Dev<R> d, Geo g, Prm<R> p, int sponge, int mul) {
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
   int mk = d.MASK[o];
+  // x*1 and x/1 leave every bit of x unchanged: cells with MASK = 1 and
+  // SPONGE = 1 need no work
+  if (mk == 1 && (!sponge || d.SPONGE[o] == (R)1)) return;
   R u = d.U[o], v = d.V[o];
   if (mul) {
     R m = (R)mk;

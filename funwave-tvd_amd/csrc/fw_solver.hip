@@ -47,6 +47,11 @@ struct fw_ctx {
   int rb = 4;
   Geo g{};
   hipStream_t stream = nullptr;
+  // fw_step runs UPDATE_MASK and WAVE_BREAKING on a second stream while the
+  // line solves run (they share no array; see do_src_huv); cur is the stream
+  // launch() uses
+  hipStream_t side = nullptr, cur = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int device = 0;
   std::vector<void *> allocs;
   size_t bytes = 0;
@@ -108,11 +113,11 @@ void launch(fw_ctx *c, const char *name, K k, dim3 grid, dim3 block, A... args) 
   if (c->prof) {
     a = get_event(c);
     b = get_event(c);
-    (void)hipEventRecord(a, c->stream);
+    (void)hipEventRecord(a, c->cur);
   }
-  hipLaunchKernelGGL(k, grid, block, 0, c->stream, args...);
+  hipLaunchKernelGGL(k, grid, block, 0, c->cur, args...);
   if (c->prof) {
-    (void)hipEventRecord(b, c->stream);
+    (void)hipEventRecord(b, c->cur);
     c->pending.push_back({name, a, b});
   }
 }
@@ -461,7 +466,14 @@ int do_huv(fw_ctx *c, int istage) {
 
 // SourceTerms fused into ESTIMATE_HUV (the whole-step path)
 template <class R>
-int do_src_huv(fw_ctx *c, int istage) {
+int do_mask(fw_ctx *c);
+template <class R>
+int do_breaking(fw_ctx *c);
+
+// overlap: UPDATE_MASK and WAVE_BREAKING follow on the second stream,
+// concurrently with the line solves
+template <class R>
+int do_src_huv(fw_ctx *c, int istage, bool overlap) {
   if (istage < 1 || istage > 3) return fail(c, FW_E_INVALID, "istage must be 1..3");
   const R alpha[3] = {(R)0, (R)3 / (R)4, (R)1 / (R)3};
   const R beta[3] = {(R)1, (R)1 / (R)4, (R)2 / (R)3};
@@ -472,7 +484,24 @@ int do_src_huv(fw_ctx *c, int istage) {
   const Geo &g = c->g;
   launch(c, "src_huv", k_huv<R, true>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), alpha[istage - 1],
          beta[istage - 1], (R)c->dt, AA);
-  return do_solve<R>(c);
+  if (!overlap) return do_solve<R>(c);
+  // UPDATE_MASK and WAVE_BREAKING read Eta, Depth, H, the dispersion slopes,
+  // the old MASK and AGE, and write MASKn, MASK9, DepthX/Y, AGEn, nu_break;
+  // the line solves and the velocity tail read the coefficients, H and the
+  // old MASK and write U, V, HU, HV, Ubar, Vbar.  Nothing is shared but the
+  // old MASK, which is double-buffered: the tail is launched first and keeps
+  // the old buffer; do_mask swaps the host pointers afterwards.
+  HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+  HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  if ((rc = do_solve<R>(c))) return rc;
+  c->cur = c->side;
+  rc = do_mask<R>(c);
+  if (rc == FW_OK) rc = do_breaking<R>(c);
+  c->cur = c->stream;
+  if (rc) return rc;
+  HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+  HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  return FW_OK;
 }
 
 // line solves and the final velocities (GET_Eta_U_V_HU_HV after the
@@ -535,9 +564,7 @@ int do_step(fw_ctx *c, int nsteps) {
     for (int st = 1; st <= 3; st++) {
       if (f.DISPERSION && (rc = do_dispersion<R>(c))) return rc;
       if ((rc = do_fluxes<R>(c))) return rc;
-      if ((rc = do_src_huv<R>(c, st))) return rc;
-      if ((rc = do_mask<R>(c))) return rc;
-      if ((rc = do_breaking<R>(c))) return rc;
+      if ((rc = do_src_huv<R>(c, st, true))) return rc;
       if ((rc = do_exchange<R>(c, f.DIRECT_SPONGE != 0))) return rc;
     }
     double m;
@@ -633,10 +660,14 @@ int fw_create(const fw_config *cfg, fw_ctx **out) {
   c->g.Ie = f.Iend;
   c->g.Jb = f.Jbeg;
   c->g.Je = f.Jend;
-  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
-    delete c;
+  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipStreamCreate(&c->side) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    fw_destroy(c);
     return FW_E_HIP;
   }
+  c->cur = c->stream;
   int rc;
   if (c->rb == 8) {
     fill_prm<double>(c);
@@ -664,6 +695,7 @@ int fw_create(const fw_config *cfg, fw_ctx **out) {
 int fw_destroy(fw_ctx *c) {
   if (!c) return FW_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->side) (void)hipStreamSynchronize(c->side);
   for (auto &p : c->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -673,6 +705,9 @@ int fw_destroy(fw_ctx *c) {
     if (c->slot_ev[k]) (void)hipEventDestroy(c->slot_ev[k]);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return FW_OK;
