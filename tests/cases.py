@@ -53,4 +53,21 @@ def tiny2d_dx15():
     return c
 
 
-CASES = {"tiny2d": tiny2d, "tiny1d": tiny1d, "island": island, "tiny2d_dx15": tiny2d_dx15}
+def config0():
+    """The reference's 1-D test at full size: 1024 x 4 cells, periodic in y,
+    WK_REG source, west sponge (the CPU-runnable benchmark configuration)."""
+    c = fwcases.beach_1d()
+    c.name = "config0"
+    return c
+
+
+def member3():
+    """Ensemble member 3 at 96 x 64 cells: planar slope with Gaussian bumps,
+    WK_REG waves on a 2-D non-periodic domain, west sponge."""
+    c = fwcases.member(3, mrange=(96, 96), nrange=(64, 64))
+    c.name = "member3"
+    return c
+
+
+CASES = {"tiny2d": tiny2d, "tiny1d": tiny1d, "island": island, "tiny2d_dx15": tiny2d_dx15,
+         "config0": config0, "member3": member3}

@@ -4,7 +4,7 @@
 For each small workload and precision: the full state after INITIALIZATION
 (every array that is not identically zero, plus the scalars) and the key
 state arrays after a number of whole time steps, all in full precision.
-Run from the repo root:  python tests/golden/make_golden.py
+Run from the repo root:  python tests/golden/make_golden.py [case ...]
 """
 import os
 import sys
@@ -23,7 +23,9 @@ KEY = ["Eta", "U", "V", "Ubar", "Vbar", "HU", "HV", "H", "MASK", "MASK9",
 PLAN = {"tiny2d": (["dp", "sp"], [1, 60, 250]),
         "tiny1d": (["dp", "sp"], [1, 400]),
         "island": (["dp", "sp"], [1, 120]),
-        "tiny2d_dx15": (["dp", "sp"], [1, 150])}
+        "tiny2d_dx15": (["dp", "sp"], [1, 150]),
+        "config0": (["dp", "sp"], [1, 200]),
+        "member3": (["dp", "sp"], [1, 150])}
 
 
 def pack(d, keys=None):
@@ -42,8 +44,10 @@ def pack(d, keys=None):
     return out
 
 
-def main():
+def main(names=None):
     for name, (precs, steps) in PLAN.items():
+        if names and name not in names:
+            continue
         for prec in precs:
             with tempfile.TemporaryDirectory() as td:
                 c = cases.CASES[name]()
@@ -63,4 +67,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -13,7 +13,7 @@ from common import STATE_KEYS, bitwise_mismatch, load_golden, oracle_available, 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not oracle_available(), reason="oracle not built")]
 
-CASES = ["tiny2d", "tiny1d", "island", "tiny2d_dx15"]
+CASES = ["tiny2d", "tiny1d", "island", "tiny2d_dx15", "config0", "member3"]
 
 
 def _solvers(name, prec, full_state=True):
@@ -100,7 +100,8 @@ def test_fused_step_matches_oracle(name, prec, full_state, gpu_available):
     g.close()
 
 
-PLAN = {"tiny2d": [1, 60, 250], "tiny1d": [1, 400], "island": [1, 120], "tiny2d_dx15": [1, 150]}
+PLAN = {"tiny2d": [1, 60, 250], "tiny1d": [1, 400], "island": [1, 120], "tiny2d_dx15": [1, 150],
+        "config0": [1, 200], "member3": [1, 150]}
 
 
 TOL = {"sp": 1e-5, "dp": 1e-10}
