@@ -69,5 +69,15 @@ def member3():
     return c
 
 
+def tiny2d_fsponge():
+    """The 2-D beach with friction and diffusion sponges in place of the
+    direct one (sources.F friction-sponge and breaking-diffusion terms)."""
+    c = tiny2d()
+    c.params.update({"DIRECT_SPONGE": False, "FRICTION_SPONGE": True, "DIFFUSION_SPONGE": True,
+                     "CDsponge": 2.0, "Csp": 0.2})
+    c.name = "tiny2d_fsponge"
+    return c
+
+
 CASES = {"tiny2d": tiny2d, "tiny1d": tiny1d, "island": island, "tiny2d_dx15": tiny2d_dx15,
-         "config0": config0, "member3": member3}
+         "config0": config0, "member3": member3, "tiny2d_fsponge": tiny2d_fsponge}
