@@ -131,6 +131,35 @@ def _solitary_coeffs(a0, h1, rdt):
     return rx * cph, bue, ae1, ae2, au
 
 
+def _tanh_sponge(M, N, dx, dy, widths, coef):
+    """CALCULATE_FRICTION_SPONGE / CALCULATE_DIFFUSION_SPONGE
+    (sponge.F:223-395, :405-580) from a zero field: coef * tanh(ri / 10) with
+    ri the number of cells inside the layer (ghosts included), west/east
+    merged by max, then south/north limited below by the merged field.  As
+    in the reference the two temporaries are not cleared between the passes."""
+    west, east, south, north = widths
+    sp = np.zeros((N, M))
+    t1 = np.zeros((N, M))
+    t2 = np.zeros((N, M))
+    I = np.arange(1, M + 1)[None, :]
+    J = np.arange(1, N + 1)[:, None]
+    if west > 0:
+        iw = int(west / dx) + 3
+        t1 = np.broadcast_to(np.maximum(coef * np.tanh(np.maximum(0.0, iw - I) / 10.0), 0.0), (N, M)).copy()
+    if east > 0:
+        iw = int(east / dx) + 3
+        t2 = np.broadcast_to(np.maximum(coef * np.tanh(np.maximum(0.0, iw - M + I) / 10.0), 0.0), (N, M)).copy()
+    sp = np.where(t1 > t2, t1, t2)
+    lim = np.where(sp > 0.0, sp, 0.0)
+    if south > 0:
+        iw = int(south / dy) + 3
+        t1 = np.maximum(coef * np.tanh(np.maximum(0.0, iw - J) / 10.0), lim)
+    if north > 0:
+        iw = int(north / dy) + 3
+        t2 = np.maximum(coef * np.tanh(np.maximum(0.0, iw - N + J) / 10.0), lim)
+    return np.where(t1 > t2, t1, t2)
+
+
 def build_state(case, precision: str = "sp") -> dict:
     """Initial state (reference names, arrays indexed [j, i]) of a case."""
     rdt = np.float64 if precision == "dp" else np.float32
@@ -165,8 +194,6 @@ def build_state(case, precision: str = "sp") -> dict:
               ETA_LIMITER=0, FIXED_DT=int("DT_fixed" in p), DT_fixed=float(p.get("DT_fixed", 0.0)),
               CrestLimit=0.0, TroughLimit=0.0,
               T_brk=20.0, WaveMaker=str(p.get("WAVEMAKER", "NONE")))
-    if st["FRICTION_SPONGE"] or st["DIFFUSION_SPONGE"]:
-        raise NotImplementedError("friction / diffusion sponge set-up is not restated here")
 
     # bathymetry, src/init.F:684-777, then mirror ghosts and face depths :789-829
     depth = np.full((N, M), 10.0)  # default before the bathymetry is read (init.F:548)
@@ -269,6 +296,11 @@ def build_state(case, precision: str = "sp") -> dict:
                                float(p.get("R_sponge", 0.85)), float(p.get("A_sponge", 5.0)))
     else:
         st["SPONGE"] = np.ones((N, M))
+    widths = tuple(float(p.get(f"Sponge_{s}_width", 0)) for s in ("west", "east", "south", "north"))
+    if st["FRICTION_SPONGE"]:  # init.F:980-989
+        st["CD_4_SPONGE"] = _tanh_sponge(M, N, dx, dy, widths, float(p.get("CDsponge", 5.0))).astype(rdt)
+    if st["DIFFUSION_SPONGE"]:  # init.F:973-978
+        st["nu_sponge"] = _tanh_sponge(M, N, dx, dy, widths, float(p.get("Csp", 0.1))).astype(rdt)
 
     # initial surface and velocity: still water, or the solitary wave of
     # INITIAL_SOLITARY_WAVE (init.F:907-910, samples.F:511-571) over all local

@@ -6,7 +6,7 @@ import pytest
 from common import load_golden
 
 
-@pytest.mark.parametrize("name", ["tiny2d", "tiny1d", "island"])
+@pytest.mark.parametrize("name", ["tiny2d", "tiny1d", "island", "member3", "tiny2d_fsponge"])
 def test_setup_matches_reference_initialisation(name):
     import cases
     import fwsetup
@@ -27,6 +27,9 @@ def test_setup_matches_reference_initialisation(name):
     np.testing.assert_array_equal(st["MASK"], ref["MASK"])
     np.testing.assert_array_equal(st["MASK9"][3:-3, 3:-3], ref["MASK9"][3:-3, 3:-3])
     np.testing.assert_allclose(st["SPONGE"], ref["SPONGE"], rtol=1e-6)
+    for k in ("CD_4_SPONGE", "nu_sponge"):
+        if k in ref:
+            np.testing.assert_allclose(st[k], ref[k], rtol=1e-12, atol=1e-15, err_msg=k)
     np.testing.assert_allclose(st["xmk_wk"][3:-3], ref["xmk_wk"][3:-3], rtol=1e-12)
     if "Cm" in ref:
         for k in ("Cm", "Sm"):
