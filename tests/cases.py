@@ -71,10 +71,11 @@ def member3():
 
 def tiny2d_fsponge():
     """The 2-D beach with friction and diffusion sponges in place of the
-    direct one (sources.F friction-sponge and breaking-diffusion terms)."""
+    direct one and with bottom friction (sources.F friction, friction-sponge
+    and breaking-diffusion terms)."""
     c = tiny2d()
     c.params.update({"DIRECT_SPONGE": False, "FRICTION_SPONGE": True, "DIFFUSION_SPONGE": True,
-                     "CDsponge": 2.0, "Csp": 0.2})
+                     "CDsponge": 2.0, "Csp": 0.2, "Cd": 0.003})
     c.name = "tiny2d_fsponge"
     return c
 
