@@ -80,5 +80,26 @@ def tiny2d_fsponge():
     return c
 
 
+def tiny2d_nobrk():
+    """The 2-D beach without the viscosity breaking model: MASK9 is the 3x3
+    wet product with the SWE_ETA_DEP switch (masks.F:165-205), and the
+    surface limiter is on (etauv_solver.F)."""
+    c = tiny2d()
+    c.params.update({"VISCOSITY_BREAKING": False, "SWE_ETA_DEP": 0.35,
+                     "ETA_LIMITER": True, "CrestLimit": 0.35, "TroughLimit": -0.3})
+    c.name = "tiny2d_nobrk"
+    return c
+
+
+def tiny2d_swe():
+    """The 2-D beach without dispersion (shallow-water equations) and with a
+    fixed time step (misc.F:283-288)."""
+    c = tiny2d()
+    c.params.update({"DISPERSION": False, "DT_fixed": 0.05})
+    c.name = "tiny2d_swe"
+    return c
+
+
 CASES = {"tiny2d": tiny2d, "tiny1d": tiny1d, "island": island, "tiny2d_dx15": tiny2d_dx15,
-         "config0": config0, "member3": member3, "tiny2d_fsponge": tiny2d_fsponge}
+         "config0": config0, "member3": member3, "tiny2d_fsponge": tiny2d_fsponge,
+         "tiny2d_nobrk": tiny2d_nobrk, "tiny2d_swe": tiny2d_swe}

@@ -26,7 +26,9 @@ PLAN = {"tiny2d": (["dp", "sp"], [1, 60, 250]),
         "tiny2d_dx15": (["dp", "sp"], [1, 150]),
         "config0": (["dp", "sp"], [1, 200]),
         "member3": (["dp", "sp"], [1, 150]),
-        "tiny2d_fsponge": (["dp", "sp"], [1, 120])}
+        "tiny2d_fsponge": (["dp", "sp"], [1, 120]),
+        "tiny2d_nobrk": (["dp", "sp"], [1, 120]),
+        "tiny2d_swe": (["dp", "sp"], [1, 120])}
 
 
 def pack(d, keys=None):

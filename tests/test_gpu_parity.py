@@ -13,7 +13,8 @@ from common import STATE_KEYS, bitwise_mismatch, load_golden, oracle_available, 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not oracle_available(), reason="oracle not built")]
 
-CASES = ["tiny2d", "tiny1d", "island", "tiny2d_dx15", "config0", "member3", "tiny2d_fsponge"]
+CASES = ["tiny2d", "tiny1d", "island", "tiny2d_dx15", "config0", "member3", "tiny2d_fsponge", "tiny2d_nobrk",
+         "tiny2d_swe"]
 
 
 def _solvers(name, prec, full_state=True):
@@ -101,7 +102,8 @@ def test_fused_step_matches_oracle(name, prec, full_state, gpu_available):
 
 
 PLAN = {"tiny2d": [1, 60, 250], "tiny1d": [1, 400], "island": [1, 120], "tiny2d_dx15": [1, 150],
-        "config0": [1, 200], "member3": [1, 150], "tiny2d_fsponge": [1, 120]}
+        "config0": [1, 200], "member3": [1, 150], "tiny2d_fsponge": [1, 120], "tiny2d_nobrk": [1, 120],
+        "tiny2d_swe": [1, 120]}
 
 
 TOL = {"sp": 1e-5, "dp": 1e-10}

@@ -15,7 +15,8 @@ from common import STATE_KEYS, bitwise_mismatch, load_golden, oracle_available, 
 pytestmark = pytest.mark.skipif(not oracle_available(), reason="oracle not built (oracle/Makefile)")
 
 PLAN = {"tiny2d": [1, 60, 250], "tiny1d": [1, 400], "island": [1, 120], "tiny2d_dx15": [1, 150],
-        "config0": [1, 200], "member3": [1, 150], "tiny2d_fsponge": [1, 120]}
+        "config0": [1, 200], "member3": [1, 150], "tiny2d_fsponge": [1, 120], "tiny2d_nobrk": [1, 120],
+        "tiny2d_swe": [1, 120]}
 
 
 @pytest.mark.parametrize("prec", ["dp", "sp"])
