@@ -1178,7 +1178,7 @@ __global__ void k_trid(Dev<R> d, Geo g, Prm<R> p, int nxblk) {
 // correctly rounded operation on the same operands as in the row-by-row
 // sweep, so results are bitwise the same.  The forward results of all but
 // the last chunk go back to the c, d arrays; the backward sweep restages
-// them in reverse order and starts from the last chunk, still in LDS.
+// them in reverse order and starts from the last two chunks, still in LDS.
 template <class R>
 struct alignas(4 * sizeof(R)) TridRow {
   R r, ca, da, keep;  // 1/a, c/a, d/a (or c, d when keep != 0)
@@ -1193,6 +1193,100 @@ struct TridTile {
   TridCD<R> o[2][L][CH + 1];   // forward output = backward input
   R z[2][L][CH + 1];           // backward output
 };
+
+// The serial lanes issue every instruction on their own (one wave per
+// SIMD), so their loops are written to carry no per-row bookkeeping: rows
+// go in groups of TG, read from LDS one group ahead into two register sets
+// used in turn, and a chunk's leftover rows (only in a line's last, partial
+// chunk) take the plain row loop.
+constexpr int TG = 8;
+
+template <class R>
+__device__ __forceinline__ void trid_fwd_row(const TridRow<R> &v, R &cprev, R &dprev, TridCD<R> *out) {
+  const R den = v.r - cprev;
+  R c = ddiv(v.ca, den);
+  R dd = ddiv(v.da - dprev, den);
+  if (v.keep != (R)0) { c = v.ca; dd = v.da; }
+  *out = TridCD<R>{c, dd};
+  cprev = c;
+  dprev = dd;
+}
+
+// Forward elimination of rows 0..kmax-1 of one line's chunk.
+template <class R, int G>
+__device__ __forceinline__ void trid_fwd_chunk(const TridRow<R> *row, TridCD<R> *out, int kmax, R &cprev,
+                                               R &dprev) {
+  const int kfull = kmax - kmax % G;
+  TridRow<R> A[G], B[G];
+  if (kfull > 0) {
+#pragma unroll
+    for (int g = 0; g < G; g++) A[g] = row[g];
+  }
+  for (int k0 = 0; k0 < kfull;) {
+    // the next group, or rows 0..G-1 again past the last whole group, so
+    // that the reads are unconditional
+    const bool nb = k0 + G < kfull;
+    const TridRow<R> *nx = nb ? row + k0 + G : row;
+#pragma unroll
+    for (int g = 0; g < G; g++) B[g] = nx[g];
+#pragma unroll
+    for (int g = 0; g < G; g++) trid_fwd_row(A[g], cprev, dprev, out + k0 + g);
+    k0 += G;
+    if (!nb) break;
+    nx = k0 + G < kfull ? row + k0 + G : row;
+#pragma unroll
+    for (int g = 0; g < G; g++) A[g] = nx[g];
+#pragma unroll
+    for (int g = 0; g < G; g++) trid_fwd_row(B[g], cprev, dprev, out + k0 + g);
+    k0 += G;
+  }
+  for (int k = kfull; k < kmax; k++) trid_fwd_row(row[k], cprev, dprev, out + k);
+}
+
+// Back substitution over rows kmax-1..0 of one line's chunk; in the line's
+// last chunk the last row is the start value z = d (tridiagnal.F:390-394).
+template <class R, int G>
+__device__ __forceinline__ void trid_bwd_chunk(const TridCD<R> *in, R *zo, int kmax, bool last, R &z) {
+  int top = kmax - 1;
+  if (last) {
+    z = in[top].d;
+    zo[top] = z;
+    top--;
+  }
+  const int cnt = top + 1;
+  const int cfull = cnt - cnt % G;
+  TridCD<R> A[G], B[G];
+  if (cfull > 0) {
+#pragma unroll
+    for (int g = 0; g < G; g++) A[g] = in[top - g];
+  }
+  for (int m = 0; m < cfull;) {
+    const bool nb = m + G < cfull;
+    const TridCD<R> *nx = nb ? in + top - m - G : in + top;
+#pragma unroll
+    for (int g = 0; g < G; g++) B[g] = nx[-g];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      z = A[g].d - A[g].c * z;
+      zo[top - m - g] = z;
+    }
+    m += G;
+    if (!nb) break;
+    nx = m + G < cfull ? in + top - m - G : in + top;
+#pragma unroll
+    for (int g = 0; g < G; g++) A[g] = nx[-g];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      z = B[g].d - B[g].c * z;
+      zo[top - m - g] = z;
+    }
+    m += G;
+  }
+  for (int k = top - cfull; k >= 0; k--) {
+    z = in[k].d - in[k].c * z;
+    zo[k] = z;
+  }
+}
 
 template <class R, int L, int CH>
 __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
@@ -1218,115 +1312,112 @@ __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
     if (xdir) return (size_t)(g.Jb + line0 + l - 1) * g.P + (size_t)(g.Ib + k - 1);
     return (size_t)(g.Jb + k - 1) * g.P + (size_t)(g.Ib + line0 + l - 1);
   };
-  // staging element q of this thread in chunk ch: (line, row in chunk), valid?
-  auto elem = [&](int q, int ch, int &l, int &k) -> bool {
+  // Staging slot q of this thread is (line sl[q], row sk[q]) of every chunk;
+  // sl[q] < 0 marks a slot past the block's lines.
+  int sl[PER], sk[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
     const int e = st + q * NS;
+    int l, k;
     if (xdir) { l = e / CH; k = e % CH; } else { l = e % L; k = e / L; }
-    return e < L * CH && l < nl && ch * CH + k < n;
-  };
-  auto fwd_stage = [&](int ch, int b) {
-    R ra[PER], rc[PER], rd[PER];
+    sl[q] = (e >= 0 && e < L * CH && l < nl) ? l : -1;
+    sk[q] = k;
+  }
+  auto ok = [&](int q, int ch) { return sl[q] >= 0 && ch * CH + sk[q] < n; };
+  // Loads are issued for every slot without branching (an unused slot reads
+  // the block's first element), so a chunk's loads are all in flight at once.
+  const size_t osafe = gix(0, 0);
+  auto addr = [&](int q, int ch) -> size_t { return ok(q, ch) ? gix(sl[q], ch * CH + sk[q]) : osafe; };
+  R ra[PER], rc[PER], rd[PER];  // one chunk loaded ahead
+  auto fwd_load = [&](int ch) {
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      int l, k;
-      if (elem(q, ch, l, k)) {
-        const size_t o = gix(l, ch * CH + k);
-        ra[q] = A[o]; rc[q] = CP[o]; rd[q] = DP[o];
-      }
+      const size_t o = addr(q, ch);
+      ra[q] = A[o]; rc[q] = CP[o]; rd[q] = DP[o];
     }
+  };
+  auto fwd_put = [&](int ch, int b) {
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      int l, k;
-      if (elem(q, ch, l, k)) {
+      if (ok(q, ch)) {
         const R a = ra[q];
         TridRow<R> v;
-        if (ch * CH + k == 0 || !(a != (R)0)) {
+        if (ch * CH + sk[q] == 0 || !(a != (R)0)) {
           v.r = (R)0; v.ca = rc[q]; v.da = rd[q]; v.keep = (R)1;
         } else {
           v.r = ddiv((R)1, a); v.ca = ddiv(rc[q], a); v.da = ddiv(rd[q], a); v.keep = (R)0;
         }
-        T.v[b][l][k] = v;
+        T.v[b][sl[q]][sk[q]] = v;
       }
     }
   };
   auto fwd_store = [&](int ch, int b) {
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      int l, k;
-      if (elem(q, ch, l, k)) {
-        const size_t o = gix(l, ch * CH + k);
-        const TridCD<R> cd = T.o[b][l][k];
+      if (ok(q, ch)) {
+        const size_t o = gix(sl[q], ch * CH + sk[q]);
+        const TridCD<R> cd = T.o[b][sl[q]][sk[q]];
         CP[o] = cd.c; DP[o] = cd.d;
       }
     }
   };
-  auto bwd_stage = [&](int ch, int b) {
-    R rc[PER], rd[PER];
+  auto bwd_load = [&](int ch) {
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      int l, k;
-      if (elem(q, ch, l, k)) { const size_t o = gix(l, ch * CH + k); rc[q] = CP[o]; rd[q] = DP[o]; }
+      const size_t o = addr(q, ch);
+      rc[q] = CP[o]; rd[q] = DP[o];
     }
+  };
+  auto bwd_put = [&](int ch, int b) {
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-      int l, k;
-      if (elem(q, ch, l, k)) T.o[b][l][k] = TridCD<R>{rc[q], rd[q]};
-    }
+    for (int q = 0; q < PER; q++)
+      if (ok(q, ch)) T.o[b][sl[q]][sk[q]] = TridCD<R>{rc[q], rd[q]};
   };
   auto bwd_store = [&](int ch, int b) {
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-      int l, k;
-      if (elem(q, ch, l, k)) Z[gix(l, ch * CH + k)] = T.z[b][l][k];
-    }
+    for (int q = 0; q < PER; q++)
+      if (ok(q, ch)) Z[gix(sl[q], ch * CH + sk[q])] = T.z[b][sl[q]][sk[q]];
   };
 
-  // ---- forward sweep
-  if (!serial) fwd_stage(0, 0);
+  // ---- forward sweep: while wave 0 eliminates chunk ch, the staging waves
+  // put chunk ch+1 (loaded during the previous chunk) into LDS, load chunk
+  // ch+2 and store the results of chunk ch-1.
+  if (!serial) {
+    fwd_load(0);
+    fwd_put(0, 0);
+    if (nch > 1) fwd_load(1);
+  }
   __syncthreads();
   R cprev = (R)0, dprev = (R)0;
   for (int ch = 0; ch < nch; ch++) {
     const int b = ch & 1;
     if (serial) {
-      if (t < nl) {
-        const int kmax = min(CH, n - ch * CH);
-        TridRow<R> cur = T.v[b][t][0];
-        for (int k = 0; k < kmax; k++) {
-          const TridRow<R> nxt = T.v[b][t][k + 1];  // slot CH is padding
-          const R den = cur.r - cprev;
-          R c = ddiv(cur.ca, den);
-          R dd = ddiv(cur.da - dprev, den);
-          if (cur.keep != (R)0) { c = cur.ca; dd = cur.da; }
-          T.o[b][t][k] = TridCD<R>{c, dd};
-          cprev = c;
-          dprev = dd;
-          cur = nxt;
-        }
-      }
+      if (t < nl) trid_fwd_chunk<R, TG>(T.v[b][t], T.o[b][t], min(CH, n - ch * CH), cprev, dprev);
     } else {
+      if (ch + 1 < nch) {
+        fwd_put(ch + 1, b ^ 1);
+        if (ch + 2 < nch) fwd_load(ch + 2);
+      }
       if (ch >= 1) fwd_store(ch - 1, b ^ 1);
-      if (ch + 1 < nch) fwd_stage(ch + 1, b ^ 1);
     }
     __syncthreads();
   }
-  // ---- backward sweep, chunks in reverse; the last chunk is in LDS
+  // ---- backward sweep, chunks in reverse.  The last two chunks' forward
+  // results are still in LDS; the staging waves put chunk ch-1 (loaded
+  // during the previous chunk) into LDS, load chunk ch-2 and store the
+  // solution of chunk ch+1.
+  if (!serial && nch >= 3) bwd_load(nch - 3);
   R z = (R)0;
   for (int ch = nch - 1; ch >= 0; ch--) {
     const int b = ch & 1;
     if (serial) {
-      if (t < nl) {
-        const int kmax = min(CH, n - ch * CH);
-        TridCD<R> cur = T.o[b][t][kmax - 1];
-        for (int k = kmax - 1; k >= 0; k--) {
-          const TridCD<R> nxt = T.o[b][t][k > 0 ? k - 1 : 0];
-          z = (ch * CH + k == n - 1) ? cur.d : cur.d - cur.c * z;
-          T.z[b][t][k] = z;
-          cur = nxt;
-        }
-      }
+      if (t < nl) trid_bwd_chunk<R, TG>(T.o[b][t], T.z[b][t], min(CH, n - ch * CH), ch == nch - 1, z);
     } else {
+      if (ch >= 1 && ch <= nch - 2) {
+        bwd_put(ch - 1, b ^ 1);
+        if (ch >= 2) bwd_load(ch - 2);
+      }
       if (ch + 1 < nch) bwd_store(ch + 1, b ^ 1);
-      if (ch >= 1) bwd_stage(ch - 1, b ^ 1);
     }
     __syncthreads();
   }
