@@ -1163,130 +1163,79 @@ __global__ void k_trid(Dev<R> d, Geo g, Prm<R> p, int nxblk) {
 }
 
 // Line solves staged through LDS.  A workgroup of 256 threads owns L lines
-// of one direction, processed in chunks of CH rows with double-buffered LDS.
-// Wave 0 runs the recurrence (lane l < L on line l) out of LDS; waves 1-3
-// meanwhile write the previous chunk's results to HBM and stage the next
-// chunk (coalesced: consecutive threads take consecutive addresses, along a
-// row for x lines and across columns for y lines), so the serial lanes only
-// wait at one barrier per chunk.
-//
-// The elimination step of tridiagnal.F:383-396 is
+// of one direction, processed in chunks of CH rows.  The elimination step of
+// tridiagnal.F:383-388 is
 //   c' = (c/a) / (1/a - c'_prev),  d' = (d/a - d'_prev) / (1/a - c'_prev)
 // (rows with a == 0, and the first row, keep c, d).  1/a, c/a and d/a do not
-// depend on the previous row, so the staging waves form them and the serial
-// lanes do only the two dependent divisions.  Every quantity is the same
-// correctly rounded operation on the same operands as in the row-by-row
-// sweep, so results are bitwise the same.  The forward results of all but
-// the last chunk go back to the c, d arrays; the backward sweep restages
-// them in reverse order and starts from the last two chunks, still in LDS.
+// depend on the previous row, so the staging waves form them; what is left
+// are two recurrences, c' alone and d' driven by the denominators of c'.
+// They run on two waves, lane l on line l:
+//   wave 0 eliminates c' of chunk ch and leaves the denominator beside it,
+//   wave 1 eliminates d' of chunk ch-1 from those denominators,
+//   waves 2-3 put chunk ch+1 (loaded during the previous chunk) into LDS,
+//   load chunk ch+2 and write the results of chunk ch-2 back to c, d
+// (coalesced: consecutive threads take consecutive addresses, along a row
+// for x lines and across columns for y lines), with one barrier per chunk
+// and three LDS buffers in rotation.  Back substitution
+// (tridiagnal.F:390-394) runs on wave 0 over the chunks in reverse, the
+// last three still in LDS, while waves 2-3 restage the others.  Every
+// quantity is the same correctly rounded operation on the same operands as
+// in the row-by-row sweep, so results are bitwise the same.  The forward
+// results of all but the last chunk go back to the c, d arrays.
 template <class R>
 struct alignas(4 * sizeof(R)) TridRow {
   R r, ca, da, keep;  // 1/a, c/a, d/a (or c, d when keep != 0)
 };
 template <class R>
 struct alignas(2 * sizeof(R)) TridCD {
-  R c, d;
+  R c, d;  // during elimination d first holds the denominator 1/a - c'_prev
 };
 template <class R, int L, int CH>
 struct TridTile {
-  TridRow<R> v[2][L][CH + 1];  // forward input
-  TridCD<R> o[2][L][CH + 1];   // forward output = backward input
+  TridRow<R> v[3][L][CH + 1];  // forward input
+  TridCD<R> o[3][L][CH + 1];   // forward output = backward input
   R z[2][L][CH + 1];           // backward output
 };
 
 // The serial lanes issue every instruction on their own (one wave per
-// SIMD), so their loops are written to carry no per-row bookkeeping: rows
-// go in groups of TG, read from LDS one group ahead into two register sets
-// used in turn, and a chunk's leftover rows (only in a line's last, partial
-// chunk) take the plain row loop.
+// SIMD), so their loops carry no per-row bookkeeping: rows go in groups of
+// G, read from LDS one group ahead into two register sets used in turn
+// (past the last whole group the reads repeat the first group, so that they
+// are unconditional), and the leftover rows of a partial chunk take a plain
+// loop.  ld(i) reads row i of the processing order, grp(i, x[G]) processes
+// rows i..i+G-1 and op(i, x) one row.
+template <int G, class Ld, class Grp, class Op>
+__device__ __forceinline__ void trid_rows(int cnt, Ld ld, Grp grp, Op op) {
+  using X = decltype(ld(0));
+  const int full = cnt - cnt % G;
+  X A[G], B[G];
+  if (full > 0) {
+#pragma unroll
+    for (int g = 0; g < G; g++) A[g] = ld(g);
+  }
+  for (int m = 0; m < full;) {
+    int nx = m + G < full ? m + G : 0;
+#pragma unroll
+    for (int g = 0; g < G; g++) B[g] = ld(nx + g);
+    grp(m, A);
+    m += G;
+    if (m >= full) break;
+    nx = m + G < full ? m + G : 0;
+#pragma unroll
+    for (int g = 0; g < G; g++) A[g] = ld(nx + g);
+    grp(m, B);
+    m += G;
+  }
+  for (int i = full; i < cnt; i++) op(i, ld(i));
+}
+
 constexpr int TG = 8;
 
 template <class R>
-__device__ __forceinline__ void trid_fwd_row(const TridRow<R> &v, R &cprev, R &dprev, TridCD<R> *out) {
-  const R den = v.r - cprev;
-  R c = ddiv(v.ca, den);
-  R dd = ddiv(v.da - dprev, den);
-  if (v.keep != (R)0) { c = v.ca; dd = v.da; }
-  *out = TridCD<R>{c, dd};
-  cprev = c;
-  dprev = dd;
-}
-
-// Forward elimination of rows 0..kmax-1 of one line's chunk.
-template <class R, int G>
-__device__ __forceinline__ void trid_fwd_chunk(const TridRow<R> *row, TridCD<R> *out, int kmax, R &cprev,
-                                               R &dprev) {
-  const int kfull = kmax - kmax % G;
-  TridRow<R> A[G], B[G];
-  if (kfull > 0) {
-#pragma unroll
-    for (int g = 0; g < G; g++) A[g] = row[g];
-  }
-  for (int k0 = 0; k0 < kfull;) {
-    // the next group, or rows 0..G-1 again past the last whole group, so
-    // that the reads are unconditional
-    const bool nb = k0 + G < kfull;
-    const TridRow<R> *nx = nb ? row + k0 + G : row;
-#pragma unroll
-    for (int g = 0; g < G; g++) B[g] = nx[g];
-#pragma unroll
-    for (int g = 0; g < G; g++) trid_fwd_row(A[g], cprev, dprev, out + k0 + g);
-    k0 += G;
-    if (!nb) break;
-    nx = k0 + G < kfull ? row + k0 + G : row;
-#pragma unroll
-    for (int g = 0; g < G; g++) A[g] = nx[g];
-#pragma unroll
-    for (int g = 0; g < G; g++) trid_fwd_row(B[g], cprev, dprev, out + k0 + g);
-    k0 += G;
-  }
-  for (int k = kfull; k < kmax; k++) trid_fwd_row(row[k], cprev, dprev, out + k);
-}
-
-// Back substitution over rows kmax-1..0 of one line's chunk; in the line's
-// last chunk the last row is the start value z = d (tridiagnal.F:390-394).
-template <class R, int G>
-__device__ __forceinline__ void trid_bwd_chunk(const TridCD<R> *in, R *zo, int kmax, bool last, R &z) {
-  int top = kmax - 1;
-  if (last) {
-    z = in[top].d;
-    zo[top] = z;
-    top--;
-  }
-  const int cnt = top + 1;
-  const int cfull = cnt - cnt % G;
-  TridCD<R> A[G], B[G];
-  if (cfull > 0) {
-#pragma unroll
-    for (int g = 0; g < G; g++) A[g] = in[top - g];
-  }
-  for (int m = 0; m < cfull;) {
-    const bool nb = m + G < cfull;
-    const TridCD<R> *nx = nb ? in + top - m - G : in + top;
-#pragma unroll
-    for (int g = 0; g < G; g++) B[g] = nx[-g];
-#pragma unroll
-    for (int g = 0; g < G; g++) {
-      z = A[g].d - A[g].c * z;
-      zo[top - m - g] = z;
-    }
-    m += G;
-    if (!nb) break;
-    nx = m + G < cfull ? in + top - m - G : in + top;
-#pragma unroll
-    for (int g = 0; g < G; g++) A[g] = nx[-g];
-#pragma unroll
-    for (int g = 0; g < G; g++) {
-      z = B[g].d - B[g].c * z;
-      zo[top - m - g] = z;
-    }
-    m += G;
-  }
-  for (int k = top - cfull; k >= 0; k--) {
-    z = in[k].d - in[k].c * z;
-    zo[k] = z;
-  }
-}
+struct TridDIn {
+  TridRow<R> v;
+  TridCD<R> o;
+};
 
 template <class R, int L, int CH>
 __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
@@ -1302,9 +1251,10 @@ __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
   R *DP = xdir ? d.xd : d.yd;
   R *Z = xdir ? d.U : d.V;
   const int t = threadIdx.x;
-  const bool serial = t < 64;  // wave 0
-  const int st = t - 64;       // staging thread index, 0..191
-  constexpr int NS = 192;
+  const int w = t >> 6;          // wave
+  const int lane = t & 63;
+  const int st = t - 128;        // staging thread index (waves 2-3), 0..127
+  constexpr int NS = 128;
   constexpr int PER = (L * CH + NS - 1) / NS;
   const int nch = (n + CH - 1) / CH;
   // global index of (line, k)
@@ -1313,7 +1263,8 @@ __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
     return (size_t)(g.Jb + k - 1) * g.P + (size_t)(g.Ib + line0 + l - 1);
   };
   // Staging slot q of this thread is (line sl[q], row sk[q]) of every chunk;
-  // sl[q] < 0 marks a slot past the block's lines.
+  // sl[q] < 0 marks a slot past the block's lines.  A chunk's c, d values
+  // are stored and later reloaded by the same thread.
   int sl[PER], sk[PER];
 #pragma unroll
   for (int q = 0; q < PER; q++) {
@@ -1378,50 +1329,98 @@ __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
     for (int q = 0; q < PER; q++)
       if (ok(q, ch)) Z[gix(sl[q], ch * CH + sk[q])] = T.z[b][sl[q]][sk[q]];
   };
+  auto rows = [&](int ch) { return min(CH, n - ch * CH); };
 
-  // ---- forward sweep: while wave 0 eliminates chunk ch, the staging waves
-  // put chunk ch+1 (loaded during the previous chunk) into LDS, load chunk
-  // ch+2 and store the results of chunk ch-1.
-  if (!serial) {
+  // ---- forward elimination, iteration it: wave 0 on chunk it, wave 1 on
+  // chunk it-1, staging as above
+  if (w >= 2) {
     fwd_load(0);
     fwd_put(0, 0);
     if (nch > 1) fwd_load(1);
   }
   __syncthreads();
   R cprev = (R)0, dprev = (R)0;
-  for (int ch = 0; ch < nch; ch++) {
-    const int b = ch & 1;
-    if (serial) {
-      if (t < nl) trid_fwd_chunk<R, TG>(T.v[b][t], T.o[b][t], min(CH, n - ch * CH), cprev, dprev);
-    } else {
-      if (ch + 1 < nch) {
-        fwd_put(ch + 1, b ^ 1);
-        if (ch + 2 < nch) fwd_load(ch + 2);
+  for (int it = 0; it <= nch; it++) {
+    if (w == 0) {
+      if (it < nch && lane < nl) {
+        const TridRow<R> *row = T.v[it % 3][lane];
+        TridCD<R> *out = T.o[it % 3][lane];
+        auto step = [&](int k, const TridRow<R> &v) {
+          const R den = v.r - cprev;
+          R c = ddiv(v.ca, den);
+          if (v.keep != (R)0) c = v.ca;
+          out[k] = TridCD<R>{c, den};
+          cprev = c;
+        };
+        auto group = [&](int k0, const TridRow<R>(&V)[TG]) {
+#pragma unroll
+          for (int q = 0; q < TG; q++) step(k0 + q, V[q]);
+        };
+        trid_rows<TG>(rows(it), [&](int k) { return row[k]; }, group, step);
       }
-      if (ch >= 1) fwd_store(ch - 1, b ^ 1);
+    } else if (w == 1) {
+      if (it >= 1 && lane < nl) {
+        const TridRow<R> *row = T.v[(it - 1) % 3][lane];
+        TridCD<R> *io = T.o[(it - 1) % 3][lane];
+        auto step = [&](int k, const TridDIn<R> &x) {
+          R dd = ddiv(x.v.da - dprev, x.o.d);
+          if (x.v.keep != (R)0) dd = x.v.da;
+          io[k].d = dd;
+          dprev = dd;
+        };
+        auto group = [&](int k0, const TridDIn<R>(&X)[TG]) {
+#pragma unroll
+          for (int q = 0; q < TG; q++) step(k0 + q, X[q]);
+        };
+        trid_rows<TG>(rows(it - 1), [&](int k) { return TridDIn<R>{row[k], io[k]}; }, group, step);
+      }
+    } else {
+      if (it + 1 < nch) {
+        fwd_put(it + 1, (it + 1) % 3);
+        if (it + 2 < nch) fwd_load(it + 2);
+      }
+      if (it >= 2 && it - 2 < nch) fwd_store(it - 2, (it - 2) % 3);
     }
     __syncthreads();
   }
-  // ---- backward sweep, chunks in reverse.  The last two chunks' forward
-  // results are still in LDS; the staging waves put chunk ch-1 (loaded
-  // during the previous chunk) into LDS, load chunk ch-2 and store the
-  // solution of chunk ch+1.
-  if (!serial && nch >= 3) bwd_load(nch - 3);
+  // ---- back substitution, chunks in reverse; chunks nch-1..nch-3 are in
+  // LDS, the staging waves put chunk ch-1 (loaded during the previous chunk)
+  // into LDS, load chunk ch-2 and store the solution of chunk ch+1.
+  if (w >= 2 && nch >= 4) bwd_load(nch - 4);
   R z = (R)0;
   for (int ch = nch - 1; ch >= 0; ch--) {
-    const int b = ch & 1;
-    if (serial) {
-      if (t < nl) trid_bwd_chunk<R, TG>(T.o[b][t], T.z[b][t], min(CH, n - ch * CH), ch == nch - 1, z);
-    } else {
-      if (ch >= 1 && ch <= nch - 2) {
-        bwd_put(ch - 1, b ^ 1);
+    if (w == 0) {
+      if (lane < nl) {
+        const TridCD<R> *in = T.o[ch % 3][lane];
+        R *zo = T.z[ch & 1][lane];
+        int top = rows(ch) - 1;
+        if (ch == nch - 1) {  // Z(Iend) = D(Iend)
+          z = in[top].d;
+          zo[top] = z;
+          top--;
+        }
+        auto back = [&](int i, const TridCD<R> &x) {
+          z = x.d - x.c * z;
+          zo[top - i] = z;
+        };
+        trid_rows<TG>(
+            top + 1, [&](int i) { return in[top - i]; },
+            [&](int i0, const TridCD<R>(&X)[TG]) {
+#pragma unroll
+              for (int q = 0; q < TG; q++) back(i0 + q, X[q]);
+            },
+            back);
+      }
+    } else if (w >= 2) {
+      if (ch >= 1 && ch <= nch - 3) {
+        bwd_put(ch - 1, (ch - 1) % 3);
         if (ch >= 2) bwd_load(ch - 2);
       }
-      if (ch + 1 < nch) bwd_store(ch + 1, b ^ 1);
+      if (ch + 1 < nch) bwd_store(ch + 1, (ch + 1) & 1);
     }
     __syncthreads();
   }
-  if (!serial) bwd_store(0, 0);
+  if (w >= 2) bwd_store(0, 0);
 }
 
 // GET_Eta_U_V_HU_HV tail (etauv_solver.F:568-593): dry cells, HU/HV, Froude
