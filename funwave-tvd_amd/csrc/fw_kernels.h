@@ -20,6 +20,25 @@ namespace fw {
 
 #define IX(i, j) ((size_t)((j)-1) * (size_t)g.P + (size_t)((i)-1))
 
+// Tile of this workgroup in a 2-D launch.  Workgroups are dispatched to the
+// 8 XCDs in turn (workgroup h of the launch, counted x-fastest, goes to XCD
+// h mod 8), and each XCD has its own L2.  Handing XCD x a contiguous run of
+// tiles in row-major order keeps a tile's left/right and upper/lower
+// neighbours, whose cells its stencils also read, on the same XCD and close
+// in time, so those cache lines come from HBM once rather than once per XCD.
+struct Tile {
+  int x, y;
+};
+__device__ __forceinline__ Tile xcd_tile() {
+  const int gx = gridDim.x;
+  const int nb = gx * gridDim.y;
+  const int h = blockIdx.x + blockIdx.y * gx;
+  const int xcd = h & 7, k = h >> 3;
+  const int per = nb >> 3, rem = nb & 7;
+  const int lin = xcd * per + min(xcd, rem) + k;
+  return Tile{lin % gx, lin / gx};
+}
+
 template <class R> struct Lit;
 template <> struct Lit<float> {
   static __host__ __device__ constexpr float grav() { return 9.81f; }
@@ -56,16 +75,20 @@ __device__ __forceinline__ double ddiv(double a, double b) { return a / b; }
 template <class R> __device__ __forceinline__ R sign1(R b) { return signbit(b) ? (R)-1 : (R)1; }
 // x / 2 and x * 0.5 are the same correctly rounded value
 template <class R> __device__ __forceinline__ R half(R x) { return x * (R)0.5; }
-// division by a grid spacing: a multiplication when the spacing is a power
-// of two (bitwise the same), a correctly rounded division otherwise
+// Division by a grid spacing: a multiplication by the reciprocal when both
+// spacings are powers of two (bitwise the same, as both are exact), a
+// correctly rounded division otherwise.  The choice is a template argument
+// of the kernels (P2), made at launch: a run-time choice puts a branch
+// around every derivative, and loads are not moved across branches, so a
+// stencil's loads would go out a few at a time.
 template <class R>
 struct Dv {
   R d, r;
-  int pow2;
+  bool pow2;
 };
 template <class R> __device__ __forceinline__ R dv(R x, const Dv<R> &s) { return s.pow2 ? x * s.r : ddiv(x, s.d); }
-template <class R> __device__ __forceinline__ Dv<R> dvx(const Prm<R> &p) { return Dv<R>{p.DX, p.rDX, p.dx_pow2}; }
-template <class R> __device__ __forceinline__ Dv<R> dvy(const Prm<R> &p) { return Dv<R>{p.DY, p.rDY, p.dy_pow2}; }
+template <bool P2, class R> __device__ __forceinline__ Dv<R> dvx(const Prm<R> &p) { return Dv<R>{p.DX, p.rDX, P2}; }
+template <bool P2, class R> __device__ __forceinline__ Dv<R> dvy(const Prm<R> &p) { return Dv<R>{p.DY, p.rDY, P2}; }
 
 // --------------------------------------------------------------------------
 // Ghost-cell fills (PHI_COLL bc.F:631-1272, PHI_COLL_VARIABLE_LENGTH
@@ -287,24 +310,34 @@ __device__ __forceinline__ R gfxy(Fn f, int i, int j, const Dv<R> &dx, const Dv<
   return dv(half(t1 - t2), dx);
 }
 
-template <class R>
+template <class R, bool P2>
 __global__ void k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const Tile tb = xcd_tile();
+  int i = 1 + tb.x * blockDim.x + threadIdx.x;
+  int j = 1 + tb.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
-  const Dv<R> dx = dvx(p), dy = dvy(p);
+  const Dv<R> dx = dvx<P2>(p), dy = dvy<P2>(p);
   const R mdf = p.MDF;
   const bool g2 = p.G2 > (R)0;
+  // Every value is formed before the first store: the output arrays could
+  // alias the inputs as far as the compiler knows, so a store would hold
+  // back the loads after it.
   // ETAT from the previous stage's P, Q (dispersion.F:103-119)
-  if (i <= g.M - 1 && j <= g.N - 1) {
-    d.ETAT[o] = -dv(d.P[IX(i + 1, j)] - d.P[IX(i, j)], dx) - dv(d.Q[IX(i, j + 1)] - d.Q[IX(i, j)], dy);
+  // (P(i+1), Q(j+1) exist for every cell, so the loads need no branch)
+  const bool has_etat = i <= g.M - 1 && j <= g.N - 1;
+  const R etat = -dv(d.P[IX(i + 1, j)] - d.P[IX(i, j)], dx) - dv(d.Q[IX(i, j + 1)] - d.Q[IX(i, j)], dy);
+  const bool keep_t = g2 && p.full_state;  // :133-142
+  R ut = (R)0, vt = (R)0;
+  if (keep_t) {
+    ut = ddiv(d.U[o] - d.U0[o], DT);
+    vt = ddiv(d.V[o] - d.V0[o], DT);
   }
-  if (g2 && p.full_state) {  // :133-142
-    d.Ut[o] = ddiv(d.U[o] - d.U0[o], DT);
-    d.Vt[o] = ddiv(d.V[o] - d.V0[o], DT);
+  if (i < g.Ib || i > g.Ie || j < g.Jb || j > g.Je) {
+    if (has_etat) d.ETAT[o] = etat;
+    if (keep_t) { d.Ut[o] = ut; d.Vt[o] = vt; }
+    return;
   }
-  if (i < g.Ib || i > g.Ie || j < g.Jb || j > g.Je) return;
   const R m9 = (R)d.MASK9[o];
   const R *U = d.U, *V = d.V, *Dp = d.Depth, *U0 = d.U0, *V0 = d.V0;
   auto DU = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * U[q]; };
@@ -314,18 +347,25 @@ __global__ void k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
   auto DUT = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * ddiv(U[q] - U0[q], DT); };
   auto DVT = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * ddiv(V[q] - V0[q], DT); };
   const bool wall = (i == g.Ib || i == g.Ie || j == g.Jb || j == g.Je);
-  d.Uxx[o] = dfxx(g, U, i, j, dx) * m9;
-  d.Uxy[o] = wall ? (R)0 : dfxy(g, U, i, j, dx, dy) * m9;
-  d.Vxy[o] = wall ? (R)0 : dfxy(g, V, i, j, dx, dy) * m9;
-  d.Vyy[o] = dfyy(g, V, i, j, dy) * m9;
-  d.DUxx[o] = gfxx<R>(DU, i, j, dx) * m9;
-  d.DUxy[o] = wall ? (R)0 : gfxy<R>(DU, i, j, dx, dy) * m9;
-  d.DVxy[o] = wall ? (R)0 : gfxy<R>(DV, i, j, dx, dy) * m9;
-  d.DVyy[o] = gfyy<R>(DV, i, j, dy) * m9;
-  if (g2 || p.SHOW_BRK) {  // read by WAVE_BREAKING
-    d.ETAx[o] = dfx(g, d.Eta, i, j, dx) * m9;
-    d.ETAy[o] = dfy(g, d.Eta, i, j, dy) * m9;
+  const R uxx = dfxx(g, U, i, j, dx) * m9;
+  const R uxy = wall ? (R)0 : dfxy(g, U, i, j, dx, dy) * m9;
+  const R vxy = wall ? (R)0 : dfxy(g, V, i, j, dx, dy) * m9;
+  const R vyy = dfyy(g, V, i, j, dy) * m9;
+  const R duxx = gfxx<R>(DU, i, j, dx) * m9;
+  const R duxy = wall ? (R)0 : gfxy<R>(DU, i, j, dx, dy) * m9;
+  const R dvxy = wall ? (R)0 : gfxy<R>(DV, i, j, dx, dy) * m9;
+  const R dvyy = gfyy<R>(DV, i, j, dy) * m9;
+  const bool slopes = g2 || p.SHOW_BRK;  // read by WAVE_BREAKING
+  R etax = (R)0, etay = (R)0;
+  if (slopes) {
+    etax = dfx(g, d.Eta, i, j, dx) * m9;
+    etay = dfy(g, d.Eta, i, j, dy) * m9;
   }
+  if (has_etat) d.ETAT[o] = etat;
+  if (keep_t) { d.Ut[o] = ut; d.Vt[o] = vt; }
+  d.Uxx[o] = uxx; d.Uxy[o] = uxy; d.Vxy[o] = vxy; d.Vyy[o] = vyy;
+  d.DUxx[o] = duxx; d.DUxy[o] = duxy; d.DVxy[o] = dvxy; d.DVyy[o] = dvyy;
+  if (slopes) { d.ETAx[o] = etax; d.ETAy[o] = etay; }
   // The first derivatives are used only at their own cell by the nonlinear
   // terms, which form them from the stencil themselves; the arrays are
   // filled only when every intermediate is kept.
@@ -355,15 +395,17 @@ __global__ void k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
 
 // CAL_DISPERSION part 2: U4, V4, U1p, V1p on all cells (:293-330) and the
 // nonlinear terms U1pp..V3 on the interior (:398-503).
-template <class R>
+template <class R, bool P2>
 __global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p, R DT) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const Tile tb = xcd_tile();
+  int i = 1 + tb.x * blockDim.x + threadIdx.x;
+  int j = 1 + tb.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
   const R B1 = p.B1, B2 = p.B2, b2 = p.b2, g2 = p.G2;
-  const Dv<R> dx = dvx(p), dy = dvy(p);
+  const Dv<R> dx = dvx<P2>(p), dy = dvy<P2>(p);
   const R c13 = (R)1 / (R)3, c12 = (R)1 / (R)2, c16 = (R)1 / (R)6;
+  R r_u4, r_v4, r_u1p, r_v1p;
   {
     R dep = d.Depth[o];
     R uxx = d.Uxx[o], vxy = d.Vxy[o], uxy = d.Uxy[o], vyy = d.Vyy[o];
@@ -379,13 +421,18 @@ __global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p, R DT) {
       u4 = u4 + g2 * m9 * (ken1 * (uxx + vxy) + ken2 * (duxx + dvxy));
       v4 = v4 + g2 * m9 * (ken1 * (uxy + vyy) + ken2 * (duxy + dvyy));
     }
-    d.U4[o] = u4;
-    d.V4[o] = v4;
-    d.U1p[o] = (R)0.5 * ((R)1 - B1) * ((R)1 - B1) * dep * dep * (uxx + vxy) + (B1 - (R)1) * dep * (duxx + dvxy);
-    d.V1p[o] = (R)0.5 * ((R)1 - B1) * ((R)1 - B1) * dep * dep * (uxy + vyy) + (B1 - (R)1) * dep * (duxy + dvyy);
+    r_u4 = u4;
+    r_v4 = v4;
+    r_u1p = (R)0.5 * ((R)1 - B1) * ((R)1 - B1) * dep * dep * (uxx + vxy) + (B1 - (R)1) * dep * (duxx + dvxy);
+    r_v1p = (R)0.5 * ((R)1 - B1) * ((R)1 - B1) * dep * dep * (uxy + vyy) + (B1 - (R)1) * dep * (duxy + dvyy);
   }
-  if (!(g2 > (R)0)) return;
-  if (i < g.Ib || i > g.Ie || j < g.Jb || j > g.Je) return;
+  // U4, V4, U1p, V1p are stored after the loads of the nonlinear terms (a
+  // store would hold back the loads after it)
+  auto store4 = [&]() { d.U4[o] = r_u4; d.V4[o] = r_v4; d.U1p[o] = r_u1p; d.V1p[o] = r_v1p; };
+  if (!(g2 > (R)0) || i < g.Ib || i > g.Ie || j < g.Jb || j > g.Je) {
+    store4();
+    return;
+  }
   // first derivatives of U, V, Ut, Vt, DU, DV, DUt, DVt at this cell, the
   // same expressions (and masking) as CAL_DISPERSION's arrays
   const R m9 = (R)d.MASK9[o], mdf = p.MDF;
@@ -450,8 +497,6 @@ __global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p, R DT) {
   R ken4 = B1 * reta * B2;
   u1pp = u1pp - ken1 * UxxVxy - ken2 * (f_Utxx + f_Vtxy) + ken3 * HUxxHVxy + ken4 * (f_DUtxx + f_DVtxy);
   v1pp = v1pp - ken1 * UxyVyy - ken2 * (f_Utxy + f_Vtyy) + ken3 * HUxyHVyy + ken4 * (f_DUtxy + f_DVtyy);
-  d.U1pp[o] = u1pp;
-  d.V1pp[o] = v1pp;
   R omB = (R)1 - B1;
   ken1 = (B1 - (R)1) * (rhx + ex) * B2;
   ken2 = (B1 - (R)1) * (rh + reta) * B2;
@@ -476,6 +521,9 @@ __global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p, R DT) {
          + (c12 * B1 * B1 - c16) * reta * reta * B2 * B2;
   d.U3[o] = -v * omega_1 - omega_0 * (ken1 * HUxyHVyy + ken2 * UxyVyy);
   d.V3[o] = u * omega_1 + omega_0 * (ken1 * HUxxHVxy + ken2 * UxxVxy);
+  d.U1pp[o] = u1pp;
+  d.V1pp[o] = v1pp;
+  store4();
 #undef A_
 }
 
@@ -909,13 +957,14 @@ __global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo 
 // FrcInsX/Y, BreakSourceX/Y, WaveMaker_Mass in the band) are written; the
 // values ESTIMATE_HUV needs are returned either way (wm: the cell's
 // WaveMaker_Mass, new in the band, else the stored one).
-template <class R>
+template <class R, bool P2>
 __device__ __forceinline__ void source_terms(const Dev<R> &d, const Geo &g, const Prm<R> &p, R AA, int i, int j,
                                              bool keep, R &sx_out, R &sy_out, R &wm_out) {
   const size_t o = IX(i, j);
-  const Dv<R> dx = dvx(p), dy = dvy(p);
+  const Dv<R> dx = dvx<P2>(p), dy = dvy<P2>(p);
   const R mdf = p.MDF, g1 = p.G1, g2 = p.G2;
   R wm;
+  bool keep_wm = false;  // the outputs kept for the caller are stored last
   if (p.wavemaker != 0 && i >= p.ilo && i <= p.ihi && j >= p.jlo && j <= p.jhi) {
     int bi = i - p.ilo, bj = j - p.jlo;
     if (p.wavemaker == 1) {
@@ -928,14 +977,13 @@ __device__ __forceinline__ void source_terms(const Dev<R> &d, const Geo &g, cons
       for (int k = 0; k < p.NumFreq; k++) s = s + AA * (d.Cm[q + k * plane] * bb[k] + d.Sm[q + k * plane] * cc[k]);
       wm = s;
     }
-    if (keep) d.WaveMaker_Mass[o] = wm;
+    keep_wm = true;
   } else {
     wm = d.WaveMaker_Mass[o];
   }
   R u = d.U[o], v = d.V[o];
   R spd = dsqrt(u * u + v * v);
   R fix = -(d.Cd[o] * u * spd);
-  if (keep) d.FrcInsX[o] = fix;
   R m9r = (R)d.MASK9[o];
   R hmax = rmax(d.H[o], mdf);
   R divpq = dv(d.P[IX(i + 1, j)] - d.P[o], dx) + dv(d.Q[IX(i, j + 1)] - d.Q[o], dy);
@@ -948,9 +996,7 @@ __device__ __forceinline__ void source_terms(const Dev<R> &d, const Geo &g, cons
                               + dv(V4[o] * (R)0.5 * (U[IX(i, j + 1)] - U[IX(i, j - 1)]), dy)
                               - g2 * m9r * (d.U1pp[o] + d.U2[o] + d.U3[o]))
          + g1 * m9r * divpq * (U4[o] - d.U1p[o]) + wm * u;
-  if (p.FRIC_SPONGE) sx = sx - d.CD_4_SPONGE[o] * u * spd * d.Depth[o];
   R fiy = -(d.Cd[o] * v * spd);
-  if (keep) d.FrcInsY[o] = fiy;
   R sy = dv(GRAV * (d.Eta[o]), dy) * (d.DepthY[IX(i, j + 1)] - d.DepthY[o]) * mk + fiy
          + g1 * m9r * hmax * (dv(u * (R)0.5 * (V4[IX(i + 1, j)] - V4[IX(i - 1, j)]), dx)
                               + dv(v * (R)0.5 * (V4[IX(i, j + 1)] - V4[IX(i, j - 1)]), dy)
@@ -958,29 +1004,46 @@ __device__ __forceinline__ void source_terms(const Dev<R> &d, const Geo &g, cons
                               + dv(V4[o] * (R)0.5 * (V[IX(i, j + 1)] - V[IX(i, j - 1)]), dy)
                               - g2 * m9r * (d.V1pp[o] + d.V2[o] + d.V3[o]))
          + g1 * m9r * divpq * (V4[o] - d.V1p[o]) + wm * v;
-  if (p.FRIC_SPONGE) sy = sy - d.CD_4_SPONGE[o] * v * spd * d.Depth[o];
-  if (p.VISC_BRK || p.DIFF_SPONGE) {
-    auto nuv = [&](int a, int b) -> float {
-      size_t q = IX(a, b);
-      float x = p.VISC_BRK ? (float)d.nu_break[q] : 0.0f;
-      if (p.DIFF_SPONGE) x = (float)((R)x + d.nu_sponge[q]);
-      return x;
-    };
-    float n0 = nuv(i, j);
-    R sxp = (R)(nuv(i + 1, j) + n0), sxm = (R)(nuv(i - 1, j) + n0);
-    R syp = (R)(nuv(i, j + 1) + n0), sym = (R)(nuv(i, j - 1) + n0);
+  if (p.FRIC_SPONGE) {
+    const R c4 = d.CD_4_SPONGE[o], dp = d.Depth[o];
+    sx = sx - c4 * u * spd * dp;
+    sy = sy - c4 * v * spd * dp;
+  }
+  R bx = (R)0, by = (R)0;
+  const bool brk = p.VISC_BRK || p.DIFF_SPONGE;
+  if (brk) {
+    // viscosity at (i,j), (i+1,j), (i-1,j), (i,j+1), (i,j-1): nu_break, plus
+    // nu_sponge, in single precision
+    const size_t qs[5] = {o, IX(i + 1, j), IX(i - 1, j), IX(i, j + 1), IX(i, j - 1)};
+    float nu[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (p.VISC_BRK) {
+#pragma unroll
+      for (int k = 0; k < 5; k++) nu[k] = (float)d.nu_break[qs[k]];
+    }
+    if (p.DIFF_SPONGE) {
+#pragma unroll
+      for (int k = 0; k < 5; k++) nu[k] = (float)((R)nu[k] + d.nu_sponge[qs[k]]);
+    }
+    float n0 = nu[0];
+    R sxp = (R)(nu[1] + n0), sxm = (R)(nu[2] + n0);
+    R syp = (R)(nu[3] + n0), sym = (R)(nu[4] + n0);
     const R *HU = d.HU, *HV = d.HV;
     const R h5x = p.h5x, h5y = p.h5y;
-    R bx = h5x * (dv(sxp * (R)1, dx) * (HU[IX(i + 1, j)] - HU[o]) - dv(sxm * (R)1, dx) * (HU[o] - HU[IX(i - 1, j)]))
+    bx = h5x * (dv(sxp * (R)1, dx) * (HU[IX(i + 1, j)] - HU[o]) - dv(sxm * (R)1, dx) * (HU[o] - HU[IX(i - 1, j)]))
            + h5y * (dv(syp * (R)1, dy) * (HU[IX(i, j + 1)] - HU[o]) - dv(sym * (R)1, dy) * (HU[o] - HU[IX(i, j - 1)]));
-    if (keep) d.BreakSourceX[o] = bx;
     sx = sx + bx;
-    R by = h5x * (dv(sxp * (R)1, dx) * (HV[IX(i + 1, j)] - HV[o]) - dv(sxm * (R)1, dx) * (HV[o] - HV[IX(i - 1, j)]))
+    by = h5x * (dv(sxp * (R)1, dx) * (HV[IX(i + 1, j)] - HV[o]) - dv(sxm * (R)1, dx) * (HV[o] - HV[IX(i - 1, j)]))
            + h5y * (dv(syp * (R)1, dy) * (HV[IX(i, j + 1)] - HV[o]) - dv(sym * (R)1, dy) * (HV[o] - HV[IX(i, j - 1)]));
-    if (keep) d.BreakSourceY[o] = by;
     sy = sy + by;
   }
   if (keep) {
+    if (keep_wm) d.WaveMaker_Mass[o] = wm;
+    d.FrcInsX[o] = fix;
+    d.FrcInsY[o] = fiy;
+    if (brk) {
+      d.BreakSourceX[o] = bx;
+      d.BreakSourceY[o] = by;
+    }
     d.SourceX[o] = sx;
     d.SourceY[o] = sy;
   }
@@ -988,13 +1051,13 @@ __device__ __forceinline__ void source_terms(const Dev<R> &d, const Geo &g, cons
   sy_out = sy;
   wm_out = wm;
 }
-template <class R>
+template <class R, bool P2>
 __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
   int i = g.Ib + blockIdx.x * blockDim.x + threadIdx.x;
   int j = g.Jb + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.Ie || j > g.Je) return;
   R sx, sy, wm;
-  source_terms(d, g, p, AA, i, j, true, sx, sy, wm);
+  source_terms<R, P2>(d, g, p, AA, i, j, true, sx, sy, wm);
 }
 
 // --------------------------------------------------------------------------
@@ -1004,28 +1067,47 @@ __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
 // FUSED: SourceTerms of the cell are formed here (source_terms) instead of
 // read back, which is the same arithmetic on the same inputs: SourceTerms
 // reads nothing ESTIMATE_HUV writes.
-template <class R, bool FUSED>
+template <class R, bool FUSED, bool P2>
 __global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const Tile tb = xcd_tile();
+  int i = 1 + tb.x * blockDim.x + threadIdx.x;
+  int j = 1 + tb.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
   const bool inner = (i >= g.Ib && i <= g.Ie && j >= g.Jb && j <= g.Je);
   const R mdf = p.MDF;
   R eta = d.Eta[o];
   if (inner) {
+    // every input of the update is loaded first, ahead of the source terms
+    // and of the first store (the compiler cannot tell the arrays apart, so
+    // a store, even one not taken, would hold back the loads after it)
+    const R *P = d.P, *Q = d.Q;
+    const R hcur = d.H[o];
+    const R p0 = P[o], p1 = P[IX(i + 1, j)], q0 = Q[o], q1 = Q[IX(i, j + 1)];
+    const R eta0 = d.Eta0[o];
+    const R fx0 = d.Fx[o], fx1 = d.Fx[IX(i + 1, j)], fy0 = d.Fy[o], fy1 = d.Fy[IX(i, j + 1)];
+    const R gx0 = d.Gx[o], gx1 = d.Gx[IX(i + 1, j)], gy0 = d.Gy[o], gy1 = d.Gy[IX(i, j + 1)];
+    const R ubar0 = d.Ubar0[o], ubar = d.Ubar[o], vbar0 = d.Vbar0[o], vbar = d.Vbar[o];
+    const R depth = d.Depth[o];
+    int mk = 0;
+    R m9 = (R)0, dxl = (R)0, dxr = (R)0, dyl = (R)0, dyr = (R)0, vxy = (R)0, dvxy = (R)0, uxy = (R)0, duxy = (R)0;
+    if (p.DISPERSION) {
+      mk = d.MASK[o];
+      m9 = (R)d.MASK9[o];
+      dxl = d.Depth[IX(i - 1, j)]; dxr = d.Depth[IX(i + 1, j)];
+      dyl = d.Depth[IX(i, j - 1)]; dyr = d.Depth[IX(i, j + 1)];
+      vxy = d.Vxy[o]; dvxy = d.DVxy[o]; uxy = d.Uxy[o]; duxy = d.DUxy[o];
+    }
     R srcx, srcy, wmm;
     if (FUSED) {
-      source_terms(d, g, p, AA, i, j, p.full_state != 0, srcx, srcy, wmm);
+      source_terms<R, P2>(d, g, p, AA, i, j, p.full_state != 0, srcx, srcy, wmm);
     } else {
       srcx = d.SourceX[o];
       srcy = d.SourceY[o];
       wmm = d.WaveMaker_Mass[o];
     }
-    const R *P = d.P, *Q = d.Q;
-    R hcur = d.H[o];
-    R pc = (R)0.5 * (P[IX(i + 1, j)] + P[o]);
-    R qc = (R)0.5 * (Q[IX(i, j + 1)] + Q[o]);
+    R pc = (R)0.5 * (p1 + p0);
+    R qc = (R)0.5 * (q1 + q0);
     if (p.full_state) {  // mean-flow diagnostics (etauv_solver.F:96-99, :247)
       d.P_center[o] = pc;
       d.Q_center[o] = qc;
@@ -1033,40 +1115,36 @@ __global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA) {
       d.V_davg[o] = ddiv(qc, rmax(hcur, mdf));
     }
     const R nl = (R)-1, nr = (R)1;
-    R fl = P[o], fr = P[IX(i + 1, j)], fb = Q[o], ft = Q[IX(i, j + 1)];
-    R r1 = -(p.rDXg * (fr * nr + fl * nl)) - p.rDYg * (ft * nr + fb * nl);
+    R r1 = -(p.rDXg * (p1 * nr + p0 * nl)) - p.rDYg * (q1 * nr + q0 * nl);
     if (p.wavemaker != 0) {
       R xmk = (R)(i - g.Ib) * p.DXg, ymk = (R)(j - g.Jb) * p.DYg;
       if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < half(p.YW)) r1 = r1 + wmm;
     }
     if (p.full_state) d.Wsurf[o] = r1;
-    eta = al * d.Eta0[o] + be * (eta + DT * r1);
+    eta = al * eta0 + be * (eta + DT * r1);
     if (p.ETA_LIMITER) {
       if (eta < p.TROUGH) eta = p.TROUGH;
       if (eta > p.CREST) eta = p.CREST;
     }
+    R r2 = -(p.rDX * (fx1 * nr + fx0 * nl)) - p.rDY * (fy1 * nr + fy0 * nl) + srcx;
+    R ub = al * ubar0 + be * (ubar + DT * r2);
+    R r3 = -(p.rDX * (gx1 * nr + gx0 * nl)) - p.rDY * (gy1 * nr + gy0 * nl) + srcy;
+    R vb = al * vbar0 + be * (vbar + DT * r3);
+    R h = eta * p.G3 + depth;
     d.Eta[o] = eta;
-    fl = d.Fx[o]; fr = d.Fx[IX(i + 1, j)]; fb = d.Fy[o]; ft = d.Fy[IX(i, j + 1)];
-    R r2 = -(p.rDX * (fr * nr + fl * nl)) - p.rDY * (ft * nr + fb * nl) + srcx;
-    R ub = al * d.Ubar0[o] + be * (d.Ubar[o] + DT * r2);
     d.Ubar[o] = ub;
-    fl = d.Gx[o]; fr = d.Gx[IX(i + 1, j)]; fb = d.Gy[o]; ft = d.Gy[IX(i, j + 1)];
-    R r3 = -(p.rDX * (fr * nr + fl * nl)) - p.rDY * (ft * nr + fb * nl) + srcy;
-    R vb = al * d.Vbar0[o] + be * (d.Vbar[o] + DT * r3);
     d.Vbar[o] = vb;
-    R h = eta * p.G3 + d.Depth[o];
     d.H[o] = h;
     if (p.DISPERSION) {
       const R b2 = p.b2;
-      const int mk = d.MASK[o];
-      const R gm = p.G1 * (R)d.MASK9[o];
-      R dep = rmax(d.Depth[o], mdf);
+      const R gm = p.G1 * m9;
+      R dep = rmax(depth, mdf);
       {
-        R depl = rmax(d.Depth[IX(i - 1, j)], mdf), depr = rmax(d.Depth[IX(i + 1, j)], mdf);
+        R depl = rmax(dxl, mdf), depr = rmax(dxr, mdf);
         R t1 = gm * (p.cxa * dep * dep + p.cxb * depl * dep);
         R t2 = (R)1 + gm * (-(p.cxc * dep * dep) - p.cxd * dep * dep);
         R t3 = gm * (p.cxa * dep * dep + p.cxb * dep * depr);
-        R t4 = ddiv(ub * (R)mk, rmax(h, mdf)) + gm * (-(p.b1h * dep * dep * d.Vxy[o]) - b2 * dep * d.DVxy[o]);
+        R t4 = ddiv(ub * (R)mk, rmax(h, mdf)) + gm * (-(p.b1h * dep * dep * vxy) - b2 * dep * dvxy);
         if (t2 != (R)0 || mk > 0) {
           d.xa[o] = ddiv(t1, t2); d.xc[o] = ddiv(t3, t2); d.xd[o] = ddiv(t4, t2);
         } else {
@@ -1074,11 +1152,11 @@ __global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA) {
         }
       }
       {
-        R depl = rmax(d.Depth[IX(i, j - 1)], mdf), depr = rmax(d.Depth[IX(i, j + 1)], mdf);
+        R depl = rmax(dyl, mdf), depr = rmax(dyr, mdf);
         R t1 = gm * (p.cya * dep * dep + p.cyb * depl * dep);
         R t2 = (R)1 + gm * (-(p.cyc * dep * dep) - p.cyd * dep * dep);
         R t3 = gm * (p.cya * dep * dep + p.cyb * dep * depr);
-        R t4 = ddiv(vb * (R)mk, rmax(h, mdf)) + gm * (-(p.b1h * dep * dep * d.Uxy[o]) - b2 * dep * d.DUxy[o]);
+        R t4 = ddiv(vb * (R)mk, rmax(h, mdf)) + gm * (-(p.b1h * dep * dep * uxy) - b2 * dep * duxy);
         if (t2 != (R)0 || mk > 0) {
           d.ya[o] = ddiv(t1, t2); d.yc[o] = ddiv(t3, t2); d.yd[o] = ddiv(t4, t2);
         } else {
@@ -1427,8 +1505,9 @@ __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
 // cap; without dispersion Ubar/Vbar := HU/HV on all cells.
 template <class R>
 __global__ void k_uv_final(Dev<R> d, Geo g, Prm<R> p) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const Tile tb = xcd_tile();
+  int i = 1 + tb.x * blockDim.x + threadIdx.x;
+  int j = 1 + tb.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
   if (i >= g.Ib && i <= g.Ie && j >= g.Jb && j <= g.Je) {
@@ -1487,8 +1566,9 @@ __device__ __forceinline__ int new_mask(const Dev<R> &d, const Geo &g, int i, in
 
 template <class R>
 __global__ void k_mask(Dev<R> d, Geo g, Prm<R> p) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const Tile tb = xcd_tile();
+  int i = 1 + tb.x * blockDim.x + threadIdx.x;
+  int j = 1 + tb.y * blockDim.y + threadIdx.y;
   if (i > g.M + 1 || j > g.N + 1) return;
   const int lo_i = g.Ib - 2, hi_i = g.Ie + 2, lo_j = g.Jb - 2, hi_j = g.Je + 2;
   // x face (i, j): written by cells i-1 (-> Depth(i)) and i (-> Depth(i-1))
@@ -1596,8 +1676,9 @@ __device__ __forceinline__ R brk_nu(const Dev<R> &d, const Geo &g, const Prm<R> 
 
 template <class R>
 __global__ void k_brk1(Dev<R> d, Geo g, Prm<R> p, R DT) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const Tile tb = xcd_tile();
+  int i = 1 + tb.x * blockDim.x + threadIdx.x;
+  int j = 1 + tb.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
   R age = d.AGE_BREAKING[o];
@@ -1673,8 +1754,9 @@ __global__ void __launch_bounds__(1024) k_brk2(Dev<R> d, Geo g, Prm<R> p, R DT) 
 // followed by SPONGE_DAMPING (sponge.F:52-66).
 template <class R>
 __global__ void k_mask_mul(Dev<R> d, Geo g, Prm<R> p, int sponge, int mul) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const Tile tb = xcd_tile();
+  int i = 1 + tb.x * blockDim.x + threadIdx.x;
+  int j = 1 + tb.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
   int mk = d.MASK[o];

@@ -144,6 +144,13 @@ const dim3 kB2(64, 4);
 // twice by vertically adjacent workgroups
 const dim3 kBT(64, 8);
 
+// both grid spacings powers of two: the stencil kernels multiply by the
+// reciprocal spacing instead of dividing (bitwise the same)
+template <class R>
+bool spacing_pow2(fw_ctx *c) {
+  return c->prm<R>().dx_pow2 && c->prm<R>().dy_pow2;
+}
+
 template <class R>
 void fill_prm(fw_ctx *c) {
   const fw_config &f = c->cfg;
@@ -401,7 +408,10 @@ template <class R>
 int do_dispersion(fw_ctx *c) {
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
-  launch(c, "disp_deriv", k_disp_deriv<R>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), (R)c->dt);
+  if (spacing_pow2<R>(c))
+    launch(c, "disp_deriv", k_disp_deriv<R, true>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), (R)c->dt);
+  else
+    launch(c, "disp_deriv", k_disp_deriv<R, false>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), (R)c->dt);
   HaloList L{}, Lf{};
   hadd(L, d.Uxx, 2); hadd(L, d.DUxx, 2); hadd(L, d.Vyy, 3); hadd(L, d.DVyy, 3);
   hadd(L, d.Uxy, 1); hadd(L, d.DUxy, 1); hadd(L, d.Vxy, 1); hadd(L, d.DVxy, 1);
@@ -416,7 +426,10 @@ int do_dispersion(fw_ctx *c) {
     hadd(L, d.ETAx, 2); hadd(L, d.ETAy, 3);
   }
   halo_exchange_list<R>(c, L, Lf);
-  launch(c, "disp_terms", k_disp_terms<R>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), (R)c->dt);
+  if (spacing_pow2<R>(c))
+    launch(c, "disp_terms", k_disp_terms<R, true>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), (R)c->dt);
+  else
+    launch(c, "disp_terms", k_disp_terms<R, false>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), (R)c->dt);
   return FW_OK;
 }
 
@@ -443,8 +456,11 @@ int do_sources(fw_ctx *c) {
   int rc = stage_wavemaker<R>(c, AA);
   if (rc) return rc;
   const Geo &g = c->g;
-  launch(c, "sources", k_sources<R>, grid2(g.Ie - g.Ib + 1, g.Je - g.Jb + 1, kB2), kB2, c->dev<R>(), g,
-         c->prm<R>(), AA);
+  const dim3 gs = grid2(g.Ie - g.Ib + 1, g.Je - g.Jb + 1, kB2);
+  if (spacing_pow2<R>(c))
+    launch(c, "sources", k_sources<R, true>, gs, kB2, c->dev<R>(), g, c->prm<R>(), AA);
+  else
+    launch(c, "sources", k_sources<R, false>, gs, kB2, c->dev<R>(), g, c->prm<R>(), AA);
   return FW_OK;
 }
 
@@ -459,7 +475,8 @@ int do_huv(fw_ctx *c, int istage) {
   const R beta[3] = {(R)1, (R)1 / (R)4, (R)2 / (R)3};
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
-  launch(c, "huv", k_huv<R, false>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), alpha[istage - 1],
+  // (the unfused update divides by no spacing)
+  launch(c, "huv", k_huv<R, false, false>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), alpha[istage - 1],
          beta[istage - 1], (R)c->dt, (R)0);
   return do_solve<R>(c);
 }
@@ -482,8 +499,12 @@ int do_src_huv(fw_ctx *c, int istage, bool overlap) {
   if (rc) return rc;
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
-  launch(c, "src_huv", k_huv<R, true>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), alpha[istage - 1],
-         beta[istage - 1], (R)c->dt, AA);
+  if (spacing_pow2<R>(c))
+    launch(c, "src_huv", k_huv<R, true, true>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), alpha[istage - 1],
+           beta[istage - 1], (R)c->dt, AA);
+  else
+    launch(c, "src_huv", k_huv<R, true, false>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), alpha[istage - 1],
+           beta[istage - 1], (R)c->dt, AA);
   if (!overlap) return do_solve<R>(c);
   // UPDATE_MASK and WAVE_BREAKING read Eta, Depth, H, the dispersion slopes,
   // the old MASK and AGE, and write MASKn, MASK9, DepthX/Y, AGEn, nu_break;
