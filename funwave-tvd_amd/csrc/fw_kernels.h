@@ -348,19 +348,21 @@ __global__ void k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
   auto DVT = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * ddiv(V[q] - V0[q], DT); };
   const bool wall = (i == g.Ib || i == g.Ie || j == g.Jb || j == g.Je);
   const R uxx = dfxx(g, U, i, j, dx) * m9;
-  const R uxy = wall ? (R)0 : dfxy(g, U, i, j, dx, dy) * m9;
-  const R vxy = wall ? (R)0 : dfxy(g, V, i, j, dx, dy) * m9;
+  // (the cross derivatives are formed on wall cells too and then dropped:
+  // their corner loads exist for every interior cell, and unconditional
+  // loads can be issued together)
+  const R uxy_s = dfxy(g, U, i, j, dx, dy) * m9, vxy_s = dfxy(g, V, i, j, dx, dy) * m9;
+  const R uxy = wall ? (R)0 : uxy_s;
+  const R vxy = wall ? (R)0 : vxy_s;
   const R vyy = dfyy(g, V, i, j, dy) * m9;
   const R duxx = gfxx<R>(DU, i, j, dx) * m9;
-  const R duxy = wall ? (R)0 : gfxy<R>(DU, i, j, dx, dy) * m9;
-  const R dvxy = wall ? (R)0 : gfxy<R>(DV, i, j, dx, dy) * m9;
+  const R duxy_s = gfxy<R>(DU, i, j, dx, dy) * m9, dvxy_s = gfxy<R>(DV, i, j, dx, dy) * m9;
+  const R duxy = wall ? (R)0 : duxy_s;
+  const R dvxy = wall ? (R)0 : dvxy_s;
   const R dvyy = gfyy<R>(DV, i, j, dy) * m9;
   const bool slopes = g2 || p.SHOW_BRK;  // read by WAVE_BREAKING
-  R etax = (R)0, etay = (R)0;
-  if (slopes) {
-    etax = dfx(g, d.Eta, i, j, dx) * m9;
-    etay = dfy(g, d.Eta, i, j, dy) * m9;
-  }
+  const R etax = dfx(g, d.Eta, i, j, dx) * m9;
+  const R etay = dfy(g, d.Eta, i, j, dy) * m9;
   if (has_etat) d.ETAT[o] = etat;
   if (keep_t) { d.Ut[o] = ut; d.Vt[o] = vt; }
   d.Uxx[o] = uxx; d.Uxy[o] = uxy; d.Vxy[o] = vxy; d.Vyy[o] = vyy;
@@ -1505,9 +1507,8 @@ __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
 // cap; without dispersion Ubar/Vbar := HU/HV on all cells.
 template <class R>
 __global__ void k_uv_final(Dev<R> d, Geo g, Prm<R> p) {
-  const Tile tb = xcd_tile();
-  int i = 1 + tb.x * blockDim.x + threadIdx.x;
-  int j = 1 + tb.y * blockDim.y + threadIdx.y;
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;  // pointwise: launch order
+  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
   if (i >= g.Ib && i <= g.Ie && j >= g.Jb && j <= g.Je) {
@@ -1754,9 +1755,8 @@ __global__ void __launch_bounds__(1024) k_brk2(Dev<R> d, Geo g, Prm<R> p, R DT) 
 // followed by SPONGE_DAMPING (sponge.F:52-66).
 template <class R>
 __global__ void k_mask_mul(Dev<R> d, Geo g, Prm<R> p, int sponge, int mul) {
-  const Tile tb = xcd_tile();
-  int i = 1 + tb.x * blockDim.x + threadIdx.x;
-  int j = 1 + tb.y * blockDim.y + threadIdx.y;
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;  // pointwise: launch order
+  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
   int mk = d.MASK[o];
