@@ -143,6 +143,10 @@ const dim3 kB2(64, 4);
 // taller tiles for kernels reading j-1/j+1 neighbours: fewer rows fetched
 // twice by vertically adjacent workgroups
 const dim3 kBT(64, 8);
+// the RK update with source terms (~100 VGPRs): four-wave workgroups fill
+// the five waves per SIMD its registers allow (two eight-wave workgroups
+// would leave one idle)
+const dim3 kBH(64, 4);
 
 // both grid spacings powers of two: the stencil kernels multiply by the
 // reciprocal spacing instead of dividing (bitwise the same)
@@ -478,7 +482,7 @@ int do_huv(fw_ctx *c, int istage) {
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
   // (the unfused update divides by no spacing)
-  launch(c, "huv", k_huv<R, false, false>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), alpha[istage - 1],
+  launch(c, "huv", k_huv<R, false, false>, grid2(g.M, g.N, kBH), kBH, d, g, c->prm<R>(), alpha[istage - 1],
          beta[istage - 1], (R)c->dt, (R)0);
   return do_solve<R>(c);
 }
@@ -502,10 +506,10 @@ int do_src_huv(fw_ctx *c, int istage, bool overlap) {
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
   if (spacing_pow2<R>(c))
-    launch(c, "src_huv", k_huv<R, true, true>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), alpha[istage - 1],
+    launch(c, "src_huv", k_huv<R, true, true>, grid2(g.M, g.N, kBH), kBH, d, g, c->prm<R>(), alpha[istage - 1],
            beta[istage - 1], (R)c->dt, AA);
   else
-    launch(c, "src_huv", k_huv<R, true, false>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), alpha[istage - 1],
+    launch(c, "src_huv", k_huv<R, true, false>, grid2(g.M, g.N, kBH), kBH, d, g, c->prm<R>(), alpha[istage - 1],
            beta[istage - 1], (R)c->dt, AA);
   if (!overlap) return do_solve<R>(c);
   // UPDATE_MASK and WAVE_BREAKING read Eta, Depth, H, the dispersion slopes,
