@@ -533,36 +533,46 @@ __global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p, R DT) {
 // FLUXES (fluxes.F:52-117).  CONSTRUCT_HO_X/Y (:1531-1754): 4th-order
 // MUSCL-TVD with compressive minmod on the 4th-order part and van Leer on
 // the 3rd-order part.
+//
+// The limiters are written without branches: both candidate values are
+// formed and a select keeps the one the reference's IF takes.  Per lane the
+// kept value is the same operation on the same operands; a divergent branch
+// would instead cost exec-mask bookkeeping (and hazard waits) on every
+// IF, which dominated these VALU-bound kernels.  A discarded candidate may be
+// a quotient by zero, which raises nothing on the GPU.
 template <class R>
 __device__ __forceinline__ R mm3(R a, R b, R c) {
   R tb = (R)2 * b, tc = (R)2 * c;
-  if (a >= (R)0) return rmax((R)0, rmin(rmin(a, tb), tc));
-  return rmin((R)0, rmax(rmax(a, tb), tc));
+  const R pos = rmax((R)0, rmin(rmin(a, tb), tc));
+  const R neg = rmin((R)0, rmax(rmax(a, tb), tc));
+  return a >= (R)0 ? pos : neg;
 }
 template <class R>
 __device__ __forceinline__ R din_val(R vm2, R vm1, R v0, R vp1, int mm2, int mm1, int m0, int mp1) {
   R t1 = vm1 - vm2, t2 = v0 - vm1, t3 = vp1 - v0;
   R d1 = mm3(t1, t2, t3), d2 = mm3(t2, t3, t1), d3 = mm3(t3, t1, t2);
-  if (mm2 == 0 || mp1 == 0) {
-    t1 = t2;
-    t3 = t2;
-    d1 = mm3(t1, t2, t3);
-    d2 = mm3(t2, t3, t1);
-    d3 = mm3(t3, t1, t2);
-  }
-  if (mm1 == 0 || m0 == 0) { d1 = (R)0; d2 = (R)0; d3 = (R)0; }
+  // next to a dry cell t1 = t3 = t2, so all three limited slopes are mm3(t2, t2, t2)
+  const R e = mm3(t2, t2, t2);
+  const bool edge = mm2 == 0 || mp1 == 0;
+  d1 = edge ? e : d1;
+  d2 = edge ? e : d2;
+  d3 = edge ? e : d3;
+  const bool dry = mm1 == 0 || m0 == 0;
+  d1 = dry ? (R)0 : d1;
+  d2 = dry ? (R)0 : d2;
+  d3 = dry ? (R)0 : d3;
   return t2 - (R)1 / (R)6 * (d3 - (R)2 * d2 + d1);
 }
 template <class R>
 __device__ __forceinline__ void vanleer(R a, R b, R &v1, R &v2, R &ta, R &tb) {
-  if (dabs(a) <= SMALL) a = SMALL * sign1(a);
-  if (dabs(b) <= SMALL) b = SMALL * sign1(b);
-  R rat = ddiv(b, a);
-  v1 = (R)0;
-  if (dabs((R)1 + rat) > SMALL) v1 = ddiv(rat + dabs(rat), (R)1 + rat);
-  rat = ddiv(a, b);
-  v2 = (R)0;
-  if (dabs((R)1 + rat) > SMALL) v2 = ddiv(rat + dabs(rat), (R)1 + rat);
+  a = dabs(a) <= SMALL ? SMALL * sign1(a) : a;
+  b = dabs(b) <= SMALL ? SMALL * sign1(b) : b;
+  const R rat = ddiv(b, a);
+  const R q1 = ddiv(rat + dabs(rat), (R)1 + rat);
+  v1 = dabs((R)1 + rat) > SMALL ? q1 : (R)0;
+  const R rat2 = ddiv(a, b);
+  const R q2 = ddiv(rat2 + dabs(rat2), (R)1 + rat2);
+  v2 = dabs((R)1 + rat2) > SMALL ? q2 : (R)0;
   ta = a;
   tb = b;
 }
@@ -591,11 +601,9 @@ __device__ __forceinline__ void muscl(const R *v, const int *mk, size_t o, ptrdi
 
 template <class R>
 __device__ __forceinline__ R hll(R sl, R sr, R fl, R fr, R ul, R ur) {
-  if (sl >= (R)0) return fl;
-  if (sr <= (R)0) return fr;
-  R o = sr * fl - sl * fr + sl * sr * (ur - ul);
-  if (dabs(sr - sl) < SMALL) return ddiv(o, SMALL);
-  return ddiv(o, sr - sl);
+  const R o = sr * fl - sl * fr + sl * sr * (ur - ul);
+  const R mid = ddiv(o, dabs(sr - sl) < SMALL ? SMALL : sr - sl);
+  return sl >= (R)0 ? fl : (sr <= (R)0 ? fr : mid);
 }
 template <class R>
 __device__ __forceinline__ R hydro(R xi, R de, R g3) {
@@ -659,17 +667,18 @@ __device__ __forceinline__ void pair_states(R vc, R D, R Dn, R &Lnext, R &Rc) {
 
 // left/right states of face c along x for field v: lanes with dok hold a
 // valid D(c), lanes with pok a valid pair for cell c
+// (o is an in-row index with two cells to its left and one to its right for
+// every lane, so the loads are unconditional; lanes without dok or pok
+// discard what they form)
 template <class R>
 __device__ __forceinline__ void recon_x(const R *v, size_t o, const int (&mk)[4], bool dok, bool pok, R &L, R &Rf) {
-  R D = (R)0, vc = (R)0;
-  if (dok) {
-    R vm2 = v[o - 2], vm1 = v[o - 1], v0 = v[o], vp1 = v[o + 1];
-    D = din_val(vm2, vm1, v0, vp1, mk[0], mk[1], mk[2], mk[3]);
-    vc = v0;
-  }
+  const R vm2 = v[o - 2], vm1 = v[o - 1], v0 = v[o], vp1 = v[o + 1];
+  const R Dv = din_val(vm2, vm1, v0, vp1, mk[0], mk[1], mk[2], mk[3]);
+  const R D = dok ? Dv : (R)0, vc = dok ? v0 : (R)0;
   R Dn = __shfl_down(D, 1);
-  R Ln = (R)0, Rc = (R)0;
-  if (pok) pair_states(vc, D, Dn, Ln, Rc);
+  R Lv, Rv;
+  pair_states(vc, D, Dn, Lv, Rv);
+  const R Ln = pok ? Lv : (R)0, Rc = pok ? Rv : (R)0;
   L = __shfl_up(Ln, 1);
   Rf = Rc;
 }
@@ -755,11 +764,11 @@ __global__ void __launch_bounds__(256) k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
     // D(c) needs cells c-2..c+1, the pair of cell c needs D(c), D(c+1)
     const bool dok = c >= g.Ib - 1 && c <= g.Ie + 2;
     const bool pok = c >= g.Ib - 1 && c <= g.Ie + 1;
-    int mk[4] = {0, 0, 0, 0};
-    if (dok) {
+    // (rows Jb..Je have cells on both sides of every lane's index: the
+    // loads need no guard)
+    int mk[4];
 #pragma unroll
-      for (int k = 0; k < 4; k++) mk[k] = d.MASK[o + k - 2];
-    }
+    for (int k = 0; k < 4; k++) mk[k] = d.MASK[o + k - 2];
     recon_x(d.U, o, mk, dok, pok, uL, uR);
     recon_x(d.V, o, mk, dok, pok, vL, vR);
     recon_x(d.HU, o, mk, dok, pok, huL, huR);
@@ -841,79 +850,108 @@ __device__ __forceinline__ void yface_finish(const Dev<R> &d, const Geo &g, cons
   d.Q[o] = Q; d.Fy[o] = Fy; d.Gy[o] = Gy;
 }
 
-// y faces, K consecutive faces of one column per thread, with no LDS and no
-// barrier: faces jb+1..jb+K need the pairs of cells jb..jb+K, which need the
-// slopes D of cells jb..jb+K+1, which read cells jb-2..jb+K+2.  The thread
-// loads those K+5 cells of each field (consecutive threads take consecutive
-// columns, so each load is one coalesced row segment) and forms everything
-// in registers; the two slopes and one pair at each end are formed again by
-// the neighbouring thread.  Same operations on the same operands as the
-// cell-by-cell reconstruction, so bitwise the same.
+// y faces: the same cell-wise reconstruction as along x, with the
+// neighbours exchanged through LDS.  A workgroup of 64 x 16 threads covers 64
+// columns and 16*K consecutive faces j of each (thread row t handles tile
+// rows t, t+16, ...); tile row r forms D(j) and the pair of cell j, takes
+// D(j+1) from row r+1 and the left state of face j from row r-1.  Rows
+// 1..16K-2 own their face, so consecutive workgroups overlap by two rows.
+constexpr int kFluxYThreadRows = 16;
+template <int K>
+struct FluxYTile {
+  static constexpr int rows = kFluxYThreadRows * K;
+  static constexpr int faces = rows - 2;
+};
+// rows per thread: 2 in FP32, 1 in FP64 (LDS holds 7 fields x rows x 64)
+template <class R>
+struct FluxYK {
+  static constexpr int K = sizeof(R) == 4 ? 2 : 1;
+};
+
 template <class R, bool DISP, int K>
-__global__ void __launch_bounds__(256) k_flux_y_col(Dev<R> d, Geo g, Prm<R> p) {
+__global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
   constexpr int NV = DISP ? 7 : 5;
-  constexpr int NC = K + 5;
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int jb = (blockIdx.y * blockDim.y + threadIdx.y) * K;
+  constexpr int ROWS = FluxYTile<K>::rows, FACES = FluxYTile<K>::faces;
+  // per field: D(j) of every tile row, then (after the pairs are formed) the
+  // left state of face j+1
+  __shared__ R sX[NV][ROWS][64];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = 1 + blockIdx.x * 64 + tx;
   const int N1 = g.N + 1;
-  if (i > g.M || jb + 1 > N1) return;
+  const ptrdiff_t s = (ptrdiff_t)g.P;
   const bool col = i >= g.Ib && i <= g.Ie;
   const R *fv[7] = {d.U, d.V, d.HU, d.HV, d.Eta, d.U4, d.V4};
-  // cell jb-2+e of this column, e = 0..NC-1.  Cells outside 1..N are read
-  // at row 1 instead: no slope that is formed uses them, and unconditional
-  // loads go out together.
-  size_t oc[NC];
-  int mk[NC];
-#pragma unroll
-  for (int e = 0; e < NC; e++) {
-    const int c = jb - 2 + e;
-    oc[e] = IX(i, c >= 1 && c <= g.N ? c : 1);
-    mk[e] = d.MASK[oc[e]];
-  }
-  R Lf[NV][K], Rf[NV][K];
-#pragma unroll
-  for (int q = 0; q < NV; q++) {
-    R w[NC];
-#pragma unroll
-    for (int e = 0; e < NC; e++) w[e] = fv[q][oc[e]];
-    R D[K + 2];  // D of cell jb+m
-#pragma unroll
-    for (int m = 0; m < K + 2; m++) {
-      const int c = jb + m;
-      const bool dok = col && c >= g.Jb - 1 && c <= g.Je + 2;
-      D[m] = dok ? din_val(w[m], w[m + 1], w[m + 2], w[m + 3], mk[m], mk[m + 1], mk[m + 2], mk[m + 3]) : (R)0;
-    }
-    R Ln[K + 1], Rc[K + 1];  // pair of cell jb+m: left state of face jb+m+1, right state of face jb+m
-#pragma unroll
-    for (int m = 0; m <= K; m++) {
-      const int c = jb + m;
-      const bool pok = col && c >= g.Jb - 1 && c <= g.Je + 1;
-      Ln[m] = (R)0;
-      Rc[m] = (R)0;
-      if (pok) pair_states(w[m + 2], D[m], D[m + 1], Ln[m], Rc[m]);
-    }
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      Lf[q][k] = Ln[k];
-      Rf[q][k] = Rc[k + 1];
-    }
-  }
+  R vc[K][NV], D[K][NV];
 #pragma unroll
   for (int k = 0; k < K; k++) {
-    const int j = jb + 1 + k;
-    if (j > N1) break;
+    const int r = ty + kFluxYThreadRows * k;
+    const int j = blockIdx.y * FACES + r;
+    // D(j) needs cells j-2..j+1
+    const bool dok = col && j >= g.Jb - 1 && j <= g.Je + 2;
+    const size_t o = IX(i <= g.M ? i : g.M, j < 1 ? 1 : (j > N1 ? N1 : j));
+    int mk[4] = {0, 0, 0, 0};
+    R w[NV][4];
+    if (dok) {
+#pragma unroll
+      for (int t = 0; t < 4; t++) mk[t] = d.MASK[o + (t - 2) * s];
+#pragma unroll
+      for (int q = 0; q < NV; q++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) w[q][t] = fv[q][o + (t - 2) * s];
+    }
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      D[k][q] = dok ? din_val(w[q][0], w[q][1], w[q][2], w[q][3], mk[0], mk[1], mk[2], mk[3]) : (R)0;
+      vc[k][q] = dok ? w[q][2] : (R)0;
+      sX[q][r][tx] = D[k][q];
+    }
+  }
+  __syncthreads();
+  R Dn[K][NV];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int r = ty + kFluxYThreadRows * k;
+#pragma unroll
+    for (int q = 0; q < NV; q++) Dn[k][q] = r + 1 < ROWS ? sX[q][r + 1][tx] : (R)0;
+  }
+  __syncthreads();
+  R Rf[K][NV];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int r = ty + kFluxYThreadRows * k;
+    const int j = blockIdx.y * FACES + r;
+    // the pair of cell j needs D(j), D(j+1)
+    const bool pok = col && j >= g.Jb - 1 && j <= g.Je + 1 && r + 1 < ROWS;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      R Lv, Rv;
+      pair_states(vc[k][q], D[k][q], Dn[k][q], Lv, Rv);
+      Rf[k][q] = pok ? Rv : (R)0;
+      sX[q][r][tx] = pok ? Lv : (R)0;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int r = ty + kFluxYThreadRows * k;
+    const int j = blockIdx.y * FACES + r;
+    const bool own = r >= 1 && r <= FACES && j <= N1 && i <= g.M;
+    if (!own) continue;
     const bool comp = col && j >= g.Jb && j <= g.Je + 1;
     const size_t o = IX(i, j);
-    R uL = Lf[0][k], uR = Rf[0][k], vL = Lf[1][k], vR = Rf[1][k], huL = Lf[2][k], huR = Rf[2][k], hvL = Lf[3][k],
-      hvR = Rf[3][k];
-    R eL = Lf[4][k], eR = Rf[4][k], u4L, u4R, v4L, v4R;
+    R Lf[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) Lf[q] = sX[q][r - 1][tx];
+    R uL = Lf[0], uR = Rf[k][0], vL = Lf[1], vR = Rf[k][1], huL = Lf[2], huR = Rf[k][2], hvL = Lf[3],
+      hvR = Rf[k][3];
+    R eL = Lf[4], eR = Rf[k][4], u4L, u4R, v4L, v4R;
     if (!comp) {
       uL = d.UyL[o]; uR = d.UyR[o]; vL = d.VyL[o]; vR = d.VyR[o];
       huL = d.HUyL[o]; huR = d.HUyR[o]; hvL = d.HVyL[o]; hvR = d.HVyR[o];
       eL = d.EtaRyL[o]; eR = d.EtaRyR[o];
     }
     if (DISP && comp) {
-      u4L = Lf[5 % NV][k]; u4R = Rf[5 % NV][k]; v4L = Lf[6 % NV][k]; v4R = Rf[6 % NV][k];
+      u4L = Lf[5 % NV]; u4R = Rf[k][5 % NV]; v4L = Lf[6 % NV]; v4R = Rf[k][6 % NV];
     } else {
       u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
     }

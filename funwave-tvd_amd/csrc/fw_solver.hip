@@ -442,16 +442,14 @@ int do_fluxes(fw_ctx *c) {
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
   const dim3 gx((g.M + 1 + kFluxXFaces - 1) / kFluxXFaces, (g.N + kB2.y - 1) / kB2.y);
-  // y faces: K faces of a column per thread, 64 x 4 threads per workgroup
-  constexpr int KY = sizeof(R) == 4 ? 4 : 2;
-  const int ty = (g.N + 1 + KY - 1) / KY;
-  const dim3 by(64, 4), gy((g.M + 63) / 64, (ty + 3) / 4);
+  constexpr int KY = FluxYK<R>::K, FY = FluxYTile<KY>::faces;
+  const dim3 gy((g.M + 63) / 64, (g.N + 1 + FY - 1) / FY), by(64, kFluxYThreadRows);
   if (c->cfg.DISPERSION) {
     launch(c, "flux_x", k_flux_x<R, true>, gx, kB2, d, g, c->prm<R>());
-    launch(c, "flux_y", k_flux_y_col<R, true, KY>, gy, by, d, g, c->prm<R>());
+    launch(c, "flux_y", k_flux_y<R, true, KY>, gy, by, d, g, c->prm<R>());
   } else {
     launch(c, "flux_x", k_flux_x<R, false>, gx, kB2, d, g, c->prm<R>());
-    launch(c, "flux_y", k_flux_y_col<R, false, KY>, gy, by, d, g, c->prm<R>());
+    launch(c, "flux_y", k_flux_y<R, false, KY>, gy, by, d, g, c->prm<R>());
   }
   return FW_OK;
 }
