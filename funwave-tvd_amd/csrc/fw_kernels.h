@@ -139,13 +139,18 @@ __device__ __forceinline__ bool halo_pos(const HaloSpec &s, int t, int mode, int
 }
 
 template <class R>
+// One array of the list per grid row (blockIdx.y): within a mode every ghost
+// value is read from interior cells only, so the arrays are independent and
+// their fills run side by side instead of one load-store round trip after
+// another.
 __global__ void k_halo(HaloList L, Geo g, int mode) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int q = 0; q < L.n; q++) {
+  {
+    const int q = blockIdx.y;
     const HaloArr A = L.a[q];
     HaloSpec s = halo_spec(g, A.spec);
     int i, j;
-    if (!halo_pos(s, t, mode, i, j)) continue;
+    if (!halo_pos(s, t, mode, i, j)) return;
     int si = i, sj = j;
     bool neg = false;
     if (mode == 1) {  // periodic y copy (raw, no sign)
@@ -1768,10 +1773,12 @@ __global__ void k_mask_mul(Dev<R> d, Geo g, Prm<R> p, int sponge, int mul) {
   int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
-  int mk = d.MASK[o];
+  // (MASK and SPONGE loaded together: one round trip)
+  const int mk = d.MASK[o];
+  const R sp = sponge ? d.SPONGE[o] : (R)1;
   // x*1 and x/1 leave every bit of x unchanged: cells with MASK = 1 and
   // SPONGE = 1 need no work
-  if (mk == 1 && (!sponge || d.SPONGE[o] == (R)1)) return;
+  if (mk == 1 && sp == (R)1) return;
   R u = d.U[o], v = d.V[o];
   if (mul) {
     R m = (R)mk;
@@ -1781,10 +1788,9 @@ __global__ void k_mask_mul(Dev<R> d, Geo g, Prm<R> p, int sponge, int mul) {
     d.HV[o] = d.HV[o] * m;
   }
   if (sponge) {
-    R s = d.SPONGE[o];
-    if ((R)mk > (R)0) d.Eta[o] = ddiv(d.Eta[o], s);
-    u = ddiv(u, s);
-    v = ddiv(v, s);
+    if ((R)mk > (R)0) d.Eta[o] = ddiv(d.Eta[o], sp);
+    u = ddiv(u, sp);
+    v = ddiv(v, sp);
   }
   d.U[o] = u;
   d.V[o] = v;

@@ -327,14 +327,16 @@ template <class R>
 void halo_exchange_list(fw_ctx *c, HaloList &L, HaloList &Lf) {
   const Geo &g = c->g;
   int cnt = 6 * (g.M + 1) + 6 * (g.N + 1) + 64;
-  dim3 b(256), gr((cnt + 255) / 256);
+  // grid rows = arrays of the list
+  dim3 b(256);
+  auto grid = [&](const HaloList &l) { return dim3((cnt + 255) / 256, l.n); };
   if (!c->cfg.PERIODIC) {
     for (int k = 0; k < Lf.n; k++) L.a[L.n++] = Lf.a[k];
-    launch(c, "halo", k_halo<R>, gr, b, L, g, 0);
+    if (L.n) launch(c, "halo", k_halo<R>, grid(L), b, L, g, 0);
   } else {
-    launch(c, "halo", k_halo<R>, gr, b, L, g, 1);
-    launch(c, "halo", k_halo<R>, gr, b, L, g, 2);
-    if (Lf.n) launch(c, "halo", k_halo<R>, gr, b, Lf, g, 0);
+    if (L.n) launch(c, "halo", k_halo<R>, grid(L), b, L, g, 1);
+    if (L.n) launch(c, "halo", k_halo<R>, grid(L), b, L, g, 2);
+    if (Lf.n) launch(c, "halo", k_halo<R>, grid(Lf), b, Lf, g, 0);
   }
 }
 
