@@ -182,9 +182,10 @@ template <class R>
 __global__ void k_dt_partial(Dev<R> d, Geo g, Prm<R> p, int copy_uv) {
   __shared__ R sm[256];
   R t3 = Lit<R>::large();
-  int n = g.M * g.N;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    int i = 1 + q % g.M, j = 1 + q / g.M;
+  // workgroups stride over rows, threads over the row (a minimum does not
+  // depend on the order it is taken in)
+  for (int j = 1 + blockIdx.x; j <= g.N; j += gridDim.x)
+  for (int i = 1 + threadIdx.x; i <= g.M; i += blockDim.x) {
     size_t o = IX(i, j);
     R u = d.U[o], v = d.V[o], h = d.H[o];
     if (copy_uv) { d.U0[o] = u; d.V0[o] = v; }
@@ -234,12 +235,11 @@ template <class R>
 __global__ void k_maxeta_partial(Dev<R> d, Geo g) {
   __shared__ R sm[256];
   R m = (R)0;
-  int nx = g.Ie - g.Ib + 1, n = nx * (g.Je - g.Jb + 1);
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    int i = g.Ib + q % nx, j = g.Jb + q / nx;
-    R a = dabs(d.Eta[IX(i, j)]);
-    if (a > m) m = a;
-  }
+  for (int j = g.Jb + blockIdx.x; j <= g.Je; j += gridDim.x)
+    for (int i = g.Ib + threadIdx.x; i <= g.Ie; i += blockDim.x) {
+      R a = dabs(d.Eta[IX(i, j)]);
+      if (a > m) m = a;
+    }
   sm[threadIdx.x] = m;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
