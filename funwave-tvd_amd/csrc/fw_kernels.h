@@ -315,8 +315,18 @@ __device__ __forceinline__ R gfxy(Fn f, int i, int j, const Dv<R> &dx, const Dv<
   return dv(half(t1 - t2), dx);
 }
 
+// Workgroup-size bounds of the stencil kernels.  Without one the compiler
+// assumes 1024-thread workgroups and caps registers at 128; the FP64
+// instances need more to keep their loads in flight, so they declare their
+// real workgroup size (64x8, 64x4).  The FP32 instances keep the default,
+// which is what sizes them for 4-5 waves per SIMD.
+template <class R>
+struct WideLB {
+  static constexpr int wg512 = sizeof(R) == 8 ? 512 : 1024;
+  static constexpr int wg256 = sizeof(R) == 8 ? 256 : 1024;
+};
 template <class R, bool P2>
-__global__ void k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
+__global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
   const Tile tb = xcd_tile();
   int i = 1 + tb.x * blockDim.x + threadIdx.x;
   int j = 1 + tb.y * blockDim.y + threadIdx.y;
@@ -403,7 +413,7 @@ __global__ void k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
 // CAL_DISPERSION part 2: U4, V4, U1p, V1p on all cells (:293-330) and the
 // nonlinear terms U1pp..V3 on the interior (:398-503).
 template <class R, bool P2>
-__global__ void k_disp_terms(Dev<R> d, Geo g, Prm<R> p, R DT) {
+__global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_terms(Dev<R> d, Geo g, Prm<R> p, R DT) {
   const Tile tb = xcd_tile();
   int i = 1 + tb.x * blockDim.x + threadIdx.x;
   int j = 1 + tb.y * blockDim.y + threadIdx.y;
@@ -1084,7 +1094,7 @@ __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
 // read back, which is the same arithmetic on the same inputs: SourceTerms
 // reads nothing ESTIMATE_HUV writes.
 template <class R, bool FUSED, bool P2>
-__global__ void k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA) {
+__global__ void __launch_bounds__(WideLB<R>::wg256) k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA) {
   const Tile tb = xcd_tile();
   int i = 1 + tb.x * blockDim.x + threadIdx.x;
   int j = 1 + tb.y * blockDim.y + threadIdx.y;
