@@ -131,7 +131,7 @@ def sync():
 
 # rocprofv3 names of the kernels behind each profiling name
 ROCPROF_NAME = {"trid": "k_trid_lds<", "src_huv": "k_huv<{r}, true,", "huv": "k_huv<{r}, false,",
-                "flux_x": "k_flux_x<", "flux_y": "k_flux_y<", "disp_deriv": "k_disp_deriv<",
+                "flux_x": "k_flux_x<", "flux_y": "k_flux_y", "disp_deriv": "k_disp_deriv<",
                 "disp_terms": "k_disp_terms<", "sources": "k_sources<", "uv_final": "k_uv_final<",
                 "mask": "k_mask<", "brk1": "k_brk1<", "mask_mul": "k_mask_mul<"}
 

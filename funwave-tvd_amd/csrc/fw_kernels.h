@@ -974,6 +974,97 @@ __global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo 
   }
 }
 
+// y faces by marching: each thread walks one column through a strip of RS
+// faces, faces jb+1..jb+RS, loading one cell (every field) per row into a
+// four-cell window; the slope D of cell c-1 is formed when cell c arrives,
+// the pair of cell c-2 when its second slope is known, and face c-2 is
+// finished from the left state the previous pair left and the right state
+// of this one.  Consecutive threads take consecutive columns (coalesced
+// row loads); no LDS, no barrier; the strip's two end slopes and one pair
+// are formed again by the neighbouring strip.  Same operations on the same
+// operands as the cell-by-cell reconstruction.
+template <class R, bool DISP, int RS>
+__global__ void __launch_bounds__(256) k_flux_y_march(Dev<R> d, Geo g, Prm<R> p) {
+  constexpr int NV = DISP ? 7 : 5;
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int jb = (blockIdx.y * blockDim.y + threadIdx.y) * RS;
+  const int N1 = g.N + 1;
+  if (i > g.M || jb + 1 > N1) return;
+  const bool col = i >= g.Ib && i <= g.Ie;
+  const R *fv[7] = {d.U, d.V, d.HU, d.HV, d.Eta, d.U4, d.V4};
+  R w[4][NV];   // cells c-3..c
+  int mk[4];
+  R Dp[NV], Lp[NV];  // slope of cell c-2, left state of face c-2 (from the pair of cell c-3)
+#pragma unroll
+  for (int q = 0; q < NV; q++) { Dp[q] = (R)0; Lp[q] = (R)0; }
+  for (int t = 0; t < 4; t++) {
+    mk[t] = 0;
+#pragma unroll
+    for (int q = 0; q < NV; q++) w[t][q] = (R)0;
+  }
+  // (kept rolled: unrolled, the window costs registers and occupancy)
+#pragma unroll 1
+  for (int t = 0; t < RS + 5; t++) {
+    const int c = jb - 2 + t;
+    // shift the window and load cell c (rows outside 1..N read row 1: no
+    // slope that is formed uses them)
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      mk[t] = mk[t + 1];
+#pragma unroll
+      for (int q = 0; q < NV; q++) w[t][q] = w[t + 1][q];
+    }
+    const size_t oc = IX(i, c >= 1 && c <= g.N ? c : 1);
+    mk[3] = d.MASK[oc];
+#pragma unroll
+    for (int q = 0; q < NV; q++) w[3][q] = fv[q][oc];
+    if (t < 3) continue;
+    // slope of cell c-1 (cells c-3..c)
+    const int cd = c - 1;
+    const bool dok = col && cd >= g.Jb - 1 && cd <= g.Je + 2;
+    R Dn[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      const R Dv = din_val(w[0][q], w[1][q], w[2][q], w[3][q], mk[0], mk[1], mk[2], mk[3]);
+      Dn[q] = dok ? Dv : (R)0;
+    }
+    if (t >= 4) {
+      // pair of cell m = c-2: right state of face m, left state of face m+1
+      const int m = c - 2;
+      const bool pok = col && m >= g.Jb - 1 && m <= g.Je + 1;
+      R Rc[NV], Ln[NV];
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        R lv, rv;
+        pair_states(w[1][q], Dp[q], Dn[q], lv, rv);
+        Ln[q] = pok ? lv : (R)0;
+        Rc[q] = pok ? rv : (R)0;
+      }
+      if (t >= 5 && m <= N1) {
+        const bool comp = col && m >= g.Jb && m <= g.Je + 1;
+        const size_t o = IX(i, m);
+        R uL = Lp[0], uR = Rc[0], vL = Lp[1], vR = Rc[1], huL = Lp[2], huR = Rc[2], hvL = Lp[3], hvR = Rc[3];
+        R eL = Lp[4], eR = Rc[4], u4L, u4R, v4L, v4R;
+        if (!comp) {
+          uL = d.UyL[o]; uR = d.UyR[o]; vL = d.VyL[o]; vR = d.VyR[o];
+          huL = d.HUyL[o]; huR = d.HUyR[o]; hvL = d.HVyL[o]; hvR = d.HVyR[o];
+          eL = d.EtaRyL[o]; eR = d.EtaRyR[o];
+        }
+        if (DISP && comp) {
+          u4L = Lp[5 % NV]; u4R = Rc[5 % NV]; v4L = Lp[6 % NV]; v4R = Rc[6 % NV];
+        } else {
+          u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
+        }
+        yface_finish(d, g, p, i, m, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
+      }
+#pragma unroll
+      for (int q = 0; q < NV; q++) Lp[q] = Ln[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NV; q++) Dp[q] = Dn[q];
+  }
+}
+
 // --------------------------------------------------------------------------
 // SourceTerms (sources.F:56-548, Cartesian): wavemaker mass source in the
 // band, then SourceX/SourceY with slope, friction, dispersion, wavemaker and

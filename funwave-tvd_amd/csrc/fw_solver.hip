@@ -444,14 +444,28 @@ int do_fluxes(fw_ctx *c) {
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
   const dim3 gx((g.M + 1 + kFluxXFaces - 1) / kFluxXFaces, (g.N + kB2.y - 1) / kB2.y);
-  constexpr int KY = FluxYK<R>::K, FY = FluxYTile<KY>::faces;
-  const dim3 gy((g.M + 63) / 64, (g.N + 1 + FY - 1) / FY), by(64, kFluxYThreadRows);
-  if (c->cfg.DISPERSION) {
+  const bool disp = c->cfg.DISPERSION;
+  if (disp)
     launch(c, "flux_x", k_flux_x<R, true>, gx, kB2, d, g, c->prm<R>());
-    launch(c, "flux_y", k_flux_y<R, true, KY>, gy, by, d, g, c->prm<R>());
-  } else {
+  else
     launch(c, "flux_x", k_flux_x<R, false>, gx, kB2, d, g, c->prm<R>());
-    launch(c, "flux_y", k_flux_y<R, false, KY>, gy, by, d, g, c->prm<R>());
+  if constexpr (sizeof(R) == 4) {
+    // FP32: columns marched in strips of 4 faces (125 VGPRs, independent waves)
+    constexpr int RS = 4;
+    const int nt = (g.N + 1 + RS - 1) / RS;
+    const dim3 by(64, 4), gy((g.M + 63) / 64, (nt + 3) / 4);
+    if (disp)
+      launch(c, "flux_y", k_flux_y_march<R, true, RS>, gy, by, d, g, c->prm<R>());
+    else
+      launch(c, "flux_y", k_flux_y_march<R, false, RS>, gy, by, d, g, c->prm<R>());
+  } else {
+    // FP64: the LDS tile (the marching window would need 216 VGPRs)
+    constexpr int KY = FluxYK<R>::K, FY = FluxYTile<KY>::faces;
+    const dim3 gy((g.M + 63) / 64, (g.N + 1 + FY - 1) / FY), by(64, kFluxYThreadRows);
+    if (disp)
+      launch(c, "flux_y", k_flux_y<R, true, KY>, gy, by, d, g, c->prm<R>());
+    else
+      launch(c, "flux_y", k_flux_y<R, false, KY>, gy, by, d, g, c->prm<R>());
   }
   return FW_OK;
 }
