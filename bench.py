@@ -155,18 +155,28 @@ def measured_traffic(kernel, rb, size):
 def cnn_roofline(n=1024, iters=20):
     """The learned correction's network (four bf16 MFMA convolutions) on the
     1024 x 1024 coarse grid of the 2048^2 workload: device time per forward
-    pass and its algorithmic FLOP rate against the dense bf16 peak."""
+    pass against its roofline.  Each layer streams its input and output
+    activations once (BYTE_PER_CELL: 8-channel bf16 input 16 B, three
+    32-channel bf16 hidden activations written and read 64 + 64 B each, fp32
+    output 12 B), so the pass does FLOP_PER_CELL / BYTE_PER_CELL ~ 100
+    FLOP/B, below the bf16 ridge (2.5e15 / 8e12 ~ 312 FLOP/B): the bound is
+    HBM; the MFMA rate is reported beside it."""
     import numpy as np
     import fwcnn
     net = fwcnn.Net(n, n, 1, fwcnn.init_params(0))
     net.forward(np.zeros((1, 5, n, n), np.float32))
     ms = net.time_forward(1, iters)
     net.close()
-    flops = fwcnn.FLOP_PER_CELL * n * n
-    return {"kernel": "k_conv x4 (bf16 MFMA 16x16x32, fp32 accumulate)", "cells": n * n,
-            "ms_per_forward": round(ms, 4), "achieved": round(flops / (ms * 1e-3) / 1e12, 2),
-            "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s", "frac": round(flops / (ms * 1e-3) / BF16_PEAK, 4),
-            "flop_per_cell": fwcnn.FLOP_PER_CELL}
+    cells = n * n
+    flops = fwcnn.FLOP_PER_CELL * cells
+    byts = fwcnn.BYTE_PER_CELL * cells
+    s = ms * 1e-3
+    return {"kernel": "k_conv x4 (bf16 MFMA 16x16x32, fp32 accumulate)", "cells": cells,
+            "ms_per_forward": round(ms, 4), "bound": "hbm",
+            "achieved": round(byts / s / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": round(byts / s / HBM_PEAK, 4), "byte_per_cell": fwcnn.BYTE_PER_CELL,
+            "mfma_achieved_tflops": round(flops / s / 1e12, 2), "mfma_peak_tflops": BF16_PEAK / 1e12,
+            "mfma_frac": round(flops / s / BF16_PEAK, 4), "flop_per_cell": fwcnn.FLOP_PER_CELL}
 
 
 def kernel_traffic_bytes(name, rb, M, N, mg, ng):

@@ -73,17 +73,39 @@ enum { MODE_RELU = 0, MODE_LAST = 1, MODE_MASK = 2 };
 // MODE_RELU: out = bf16(relu(acc + bias)), 32 channels
 // MODE_LAST: out = fp32 planar [b][3][H][W] = acc + bias (channels 0..2)
 // MODE_MASK: out = bf16(acc) * (act > 0), 32 channels (gradient through ReLU)
+//
+// A workgroup (4 waves) computes rows y0..y0+3, pixels x0..x0+63.  The 6
+// input rows and 66 pixels they read are staged into LDS once (16-B loads;
+// outside the padded grid: zeros), and every tap's A fragment is an LDS
+// read: each input pixel crosses L1 once instead of nine times.  The pixel
+// stride in LDS is 80 B for 32 channels, so the 16 lanes of a fragment read
+// hit distinct banks.
 template <int CIN, int NT, int MODE>
 __global__ void __launch_bounds__(256) k_conv(const bf16 *__restrict__ in, const bf16 *__restrict__ wpk,
                                               const float *__restrict__ bias, void *__restrict__ out,
                                               const bf16 *__restrict__ act, Grid g) {
   constexpr int NS = CIN == 32 ? 9 : 3;
+  constexpr int TW = 66, TR = 6;           // staged pixels per row, staged rows
+  constexpr int LPX = CIN == 32 ? 40 : 8;  // LDS pixel stride (bf16)
+  constexpr int CH16 = CIN / 8;            // 16-B chunks per pixel
+  __shared__ __attribute__((aligned(16))) bf16 stg[TR * TW * LPX];
   __shared__ __attribute__((aligned(16))) bf16 tile[4][16][kC + 8];  // per wave: 16 px x 32 ch (+pad)
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int row = l & 15, grp = l >> 4;
-  const int y = blockIdx.y * 4 + wave;
+  const int y0 = blockIdx.y * 4, x0b = blockIdx.x * 64;
+  const int y = y0 + wave;
   const int b = blockIdx.z;
   const bool live = y < g.H;
+  // padded rows y0..y0+5 (pixel rows y0-1..y0+4), padded columns x0b..x0b+65
+  for (int e = threadIdx.x; e < TR * TW * CH16; e += 256) {
+    const int c = e % CH16, px = (e / CH16) % TW, r = e / (CH16 * TW);
+    const int gy = y0 + r, gx = x0b + px;
+    uint4 v{0u, 0u, 0u, 0u};
+    if (gy < g.H + 2 && gx < g.W + 2)
+      v = *(const uint4 *)(in + ((size_t)b * g.PS + (size_t)gy * (g.W + 2) + gx) * CIN + 8 * c);
+    *(uint4 *)&stg[(r * TW + px) * LPX + 8 * c] = v;
+  }
+  __syncthreads();
   bf16x8 bw[NS][NT];
 #pragma unroll
   for (int s = 0; s < NS; s++)
@@ -92,30 +114,38 @@ __global__ void __launch_bounds__(256) k_conv(const bf16 *__restrict__ in, const
   float bs[NT];
 #pragma unroll
   for (int nt = 0; nt < NT; nt++) bs[nt] = (MODE == MODE_MASK) ? 0.f : bias[nt * 16 + row];
-  for (int mt = 0; mt < 4; mt++) {
-    const int x0 = blockIdx.x * 64 + mt * 16;
-    f32x4 acc[NT];
+  // taps outer, the 4 M-tiles inner: 4 x NT independent accumulation chains
+  // keep the matrix pipe busy
+  f32x4 accs[4][NT];
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-    if (live) {
-      const int x = x0 + row;
+  for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-      for (int s = 0; s < NS; s++) {
-        const int tap = CIN == 32 ? s : 4 * s + grp;
-        const int choff = CIN == 32 ? 8 * grp : 0;
+    for (int nt = 0; nt < NT; nt++) accs[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  if (live) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const int tap = CIN == 32 ? s : 4 * s + grp;
+      const int choff = CIN == 32 ? 8 * grp : 0;
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
         bf16x8 a;
         if (CIN == 32 || tap < 9) {
           const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-          const size_t p = (size_t)b * g.PS + (size_t)(y + 1 + dy) * (g.W + 2) + (x + 1 + dx);
-          a = *(const bf16x8 *)(in + p * CIN + choff);
+          a = *(const bf16x8 *)&stg[((wave + 1 + dy) * TW + (mt * 16 + row + 1 + dx)) * LPX + choff];
         } else {
 #pragma unroll
           for (int e = 0; e < 8; e++) a[e] = (__bf16)0.f;
         }
 #pragma unroll
-        for (int nt = 0; nt < NT; nt++) acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[s][nt], acc[nt], 0, 0, 0);
+        for (int nt = 0; nt < NT; nt++)
+          accs[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[s][nt], accs[mt][nt], 0, 0, 0);
       }
     }
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    const int x0 = blockIdx.x * 64 + mt * 16;
+    const f32x4 *acc = accs[mt];
     if (MODE == MODE_LAST) {
       if (live && row < kCout) {
         float *o = (float *)out;
@@ -136,7 +166,10 @@ __global__ void __launch_bounds__(256) k_conv(const bf16 *__restrict__ in, const
         if (MODE == MODE_RELU) v = v > 0.f ? v : 0.f;
         tile[wave][4 * grp + r][nt * 16 + row] = f2bf(v);
       }
-    __syncthreads();
+    // the tile is this wave's own: its LDS operations complete in order, so
+    // a wavefront-scope fence (no workgroup barrier) orders write and read
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
     const int q = l >> 2, cg = l & 3;
     const int x = x0 + q;
     if (live && x < g.W) {
@@ -157,7 +190,8 @@ __global__ void __launch_bounds__(256) k_conv(const bf16 *__restrict__ in, const
       }
       *(uint4 *)((bf16 *)out + p * kC + 8 * cg) = v;
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
 }
 

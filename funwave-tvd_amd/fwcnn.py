@@ -22,6 +22,10 @@ EXPORTS = ["fw_cnn_create", "fw_cnn_destroy", "fw_cnn_strerror", "fw_cnn_set_par
            "fw_cnn_time_forward"]
 # multiply-adds x 2 per cell of the four layers (algorithmic, unpadded channels)
 FLOP_PER_CELL = 2 * 9 * (5 * 32 + 32 * 32 + 32 * 32 + 32 * 3)
+# HBM bytes per cell of a forward pass, each activation written and read once:
+# 8-channel bf16 input 16, three 32-channel bf16 activations 3 x (64 + 64),
+# fp32 output 3 x 4
+BYTE_PER_CELL = 16 + 3 * (64 + 64) + 12
 # feature scales: eta / 1 m, u, v / 1 m/s, depth / 10 m, mask
 FEATURE_SCALE = np.array([1.0, 1.0, 1.0, 0.1], np.float32)
 # outputs are corrections in units of 1 mm (eta) and 1 mm/s (u, v)
