@@ -38,15 +38,15 @@ KERNEL_TRAFFIC = {
     "flux_x": (8, 2, 3, "xface"),        # U V HU HV Eta U4 V4 DepthX, MASK MASK9 -> P Fx Gx
     "flux_y": (8, 2, 3, "yface"),
     "disp_deriv": (6, 1, 11, "cell"),    # U V Depth Eta P Q, MASK9 -> 8 second derivatives, ETAT ETAx ETAy
-    "disp_terms": (17, 1, 10, "cell"),   # 8 second derivatives, ETAT ETAx ETAy, U V U0 V0 Depth Eta
-                                         #   -> U4 V4 U1p V1p U1pp V1pp U2 V2 U3 V3
+    "disp_terms": (17, 1, 6, "cell"),    # 8 second derivatives, ETAT ETAx ETAy, U V U0 V0 Depth Eta
+                                         #   -> U4 V4 U1p V1p and the sums U1pp+U2+U3, V1pp+V2+V3
     "sources": (24, 2, 2, "inner"),
     "huv": (24, 2, 12, "cell"),
     # SourceTerms fused into ESTIMATE_HUV: U V U4 V4 Cd H P Q Eta DepthX DepthY
-    # U1p V1p U1pp V1pp U2 V2 U3 V3 nu_break HU HV Eta0 Ubar0 Vbar0 Ubar Vbar
-    # Fx Fy Gx Gy Depth Uxy DUxy Vxy DVxy, MASK MASK9
+    # U1p V1p, the two dispersive sums, nu_break HU HV Eta0 Ubar0 Vbar0 Ubar
+    # Vbar Fx Fy Gx Gy Depth Uxy DUxy Vxy DVxy, MASK MASK9
     #   -> Eta Ubar Vbar H and the six line-solve coefficient arrays
-    "src_huv": (36, 2, 10, "cell"),
+    "src_huv": (32, 2, 10, "cell"),
     "trid": (6, 0, 2, "inner"),
     "uv_final": (3, 1, 4, "cell"),
     "mask": (4, 2, 2, "cell"),

@@ -58,6 +58,9 @@ struct Dev {
   R *wk_stage;     // per-stage host-evaluated factors (sin per row / cos,sin per freq)
   // line-solve coefficients
   R *xa, *xc, *xd, *ya, *yc, *yd, *c1, *y1, *y2;
+  // (U1pp + U2) + U3 and (V1pp + V2) + V3, the only form in which the fused
+  // RK update reads the nonlinear dispersive terms
+  R *sumU, *sumV;
   // reductions and breaking candidates
   R *red;          // block partials
   double *red64;   // FP64 partials (stats)

@@ -523,11 +523,11 @@ __global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_terms(Dev<R> d, Geo g
   R u = A_(U, i, j), v = A_(V, i, j);
   R ux = f_Ux, vx = f_Vx, uy = f_Uy, vy = f_Vy;
   R dux = f_DUx, dvy = f_DVy;
-  d.U2[o] = ken1 * (u * HUxxHVxy + v * HUxyHVyy) + ken2 * (ux * HUxxHVxy + u * HUxxHVxy_x + vx * HUxyHVyy + v * HUxyHVyy_x)
+  const R u2 = ken1 * (u * HUxxHVxy + v * HUxyHVyy) + ken2 * (ux * HUxxHVxy + u * HUxxHVxy_x + vx * HUxyHVyy + v * HUxyHVyy_x)
             + ken3 * (u * UxxVxy + v * UxyVyy) + ken4 * (ux * UxxVxy + u * UxxVxy_x + vx * UxyVyy + v * UxyVyy_x)
             + B2 * B2 * (dux + dvy + reta * B2 * (ux + vy)) * (HUxxHVxy + ex * B2 * (ux + vy) + reta * B2 * UxxVxy);
   ken1 = (B1 - (R)1) * (rhy + ey) * B2;
-  d.V2[o] = ken1 * (u * HUxxHVxy + v * HUxyHVyy) + ken2 * (uy * HUxxHVxy + u * HUxxHVxy_y + vy * HUxyHVyy + v * HUxyHVyy_y)
+  const R v2 = ken1 * (u * HUxxHVxy + v * HUxyHVyy) + ken2 * (uy * HUxxHVxy + u * HUxxHVxy_y + vy * HUxyHVyy + v * HUxyHVyy_y)
             + ken5 * (u * UxxVxy + v * UxyVyy) + ken4 * (uy * UxxVxy + u * UxxVxy_y + vy * UxyVyy + v * UxyVyy_y)
             + B2 * B2 * (dux + dvy + reta * B2 * (ux + vy)) * (HUxyHVyy + ey * B2 * (ux + vy) + reta * B2 * UxyVyy);
   R omega_0 = vx - uy;
@@ -536,10 +536,17 @@ __global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_terms(Dev<R> d, Geo g
   ken1 = ((B1 - c12) * (reta + rh) * B2);
   ken2 = (c13 - B1 + (R)0.5 * B1 * B1) * rh * rh * B2 * B2 + (c16 - B1 + B1 * B1) * rh * reta * B2
          + (c12 * B1 * B1 - c16) * reta * reta * B2 * B2;
-  d.U3[o] = -v * omega_1 - omega_0 * (ken1 * HUxyHVyy + ken2 * UxyVyy);
-  d.V3[o] = u * omega_1 + omega_0 * (ken1 * HUxxHVxy + ken2 * UxxVxy);
-  d.U1pp[o] = u1pp;
-  d.V1pp[o] = v1pp;
+  const R u3 = -v * omega_1 - omega_0 * (ken1 * HUxyHVyy + ken2 * UxyVyy);
+  const R v3 = u * omega_1 + omega_0 * (ken1 * HUxxHVxy + ken2 * UxxVxy);
+  if (p.full_state) {
+    d.U1pp[o] = u1pp; d.V1pp[o] = v1pp;
+    d.U2[o] = u2; d.V2[o] = v2;
+    d.U3[o] = u3; d.V3[o] = v3;
+  }
+  // the sums as SourceTerms forms them (sources.F:254, left to right); the
+  // arrays themselves are kept only with every intermediate
+  d.sumU[o] = u1pp + u2 + u3;
+  d.sumV[o] = v1pp + v2 + v3;
   store4();
 #undef A_
 }
@@ -1076,7 +1083,7 @@ __global__ void __launch_bounds__(256) k_flux_y_march(Dev<R> d, Geo g, Prm<R> p)
 // WaveMaker_Mass, new in the band, else the stored one).
 template <class R, bool P2>
 __device__ __forceinline__ void source_terms(const Dev<R> &d, const Geo &g, const Prm<R> &p, R AA, int i, int j,
-                                             bool keep, R &sx_out, R &sy_out, R &wm_out) {
+                                             bool keep, bool sums, R &sx_out, R &sy_out, R &wm_out) {
   const size_t o = IX(i, j);
   const Dv<R> dx = dvx<P2>(p), dy = dvy<P2>(p);
   const R mdf = p.MDF, g1 = p.G1, g2 = p.G2;
@@ -1105,13 +1112,17 @@ __device__ __forceinline__ void source_terms(const Dev<R> &d, const Geo &g, cons
   R hmax = rmax(d.H[o], mdf);
   R divpq = dv(d.P[IX(i + 1, j)] - d.P[o], dx) + dv(d.Q[IX(i, j + 1)] - d.Q[o], dy);
   const R *U4 = d.U4, *V4 = d.V4, *U = d.U, *V = d.V;
+  // (U1pp + U2) + U3 (sources.F:254): with every intermediate kept, from the
+  // MODULE GLOBAL arrays; otherwise the sums CAL_DISPERSION's kernel formed
+  const R dsum_u = sums ? d.sumU[o] : d.U1pp[o] + d.U2[o] + d.U3[o];
+  const R dsum_v = sums ? d.sumV[o] : d.V1pp[o] + d.V2[o] + d.V3[o];
   R mk = (R)d.MASK[o];
   R sx = dv(GRAV * (d.Eta[o]), dx) * (d.DepthX[IX(i + 1, j)] - d.DepthX[o]) * mk + fix
          + g1 * m9r * hmax * (dv(u * (R)0.5 * (U4[IX(i + 1, j)] - U4[IX(i - 1, j)]), dx)
                               + dv(v * (R)0.5 * (U4[IX(i, j + 1)] - U4[IX(i, j - 1)]), dy)
                               + dv(U4[o] * (R)0.5 * (U[IX(i + 1, j)] - U[IX(i - 1, j)]), dx)
                               + dv(V4[o] * (R)0.5 * (U[IX(i, j + 1)] - U[IX(i, j - 1)]), dy)
-                              - g2 * m9r * (d.U1pp[o] + d.U2[o] + d.U3[o]))
+                              - g2 * m9r * dsum_u)
          + g1 * m9r * divpq * (U4[o] - d.U1p[o]) + wm * u;
   R fiy = -(d.Cd[o] * v * spd);
   R sy = dv(GRAV * (d.Eta[o]), dy) * (d.DepthY[IX(i, j + 1)] - d.DepthY[o]) * mk + fiy
@@ -1119,7 +1130,7 @@ __device__ __forceinline__ void source_terms(const Dev<R> &d, const Geo &g, cons
                               + dv(v * (R)0.5 * (V4[IX(i, j + 1)] - V4[IX(i, j - 1)]), dy)
                               + dv(U4[o] * (R)0.5 * (V[IX(i + 1, j)] - V[IX(i - 1, j)]), dx)
                               + dv(V4[o] * (R)0.5 * (V[IX(i, j + 1)] - V[IX(i, j - 1)]), dy)
-                              - g2 * m9r * (d.V1pp[o] + d.V2[o] + d.V3[o]))
+                              - g2 * m9r * dsum_v)
          + g1 * m9r * divpq * (V4[o] - d.V1p[o]) + wm * v;
   if (p.FRIC_SPONGE) {
     const R c4 = d.CD_4_SPONGE[o], dp = d.Depth[o];
@@ -1174,7 +1185,7 @@ __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
   int j = g.Jb + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.Ie || j > g.Je) return;
   R sx, sy, wm;
-  source_terms<R, P2>(d, g, p, AA, i, j, true, sx, sy, wm);
+  source_terms<R, P2>(d, g, p, AA, i, j, true, p.full_state == 0, sx, sy, wm);
 }
 
 // --------------------------------------------------------------------------
@@ -1217,7 +1228,7 @@ __global__ void __launch_bounds__(WideLB<R>::wg256) k_huv(Dev<R> d, Geo g, Prm<R
     }
     R srcx, srcy, wmm;
     if (FUSED) {
-      source_terms<R, P2>(d, g, p, AA, i, j, p.full_state != 0, srcx, srcy, wmm);
+      source_terms<R, P2>(d, g, p, AA, i, j, p.full_state != 0, p.full_state == 0, srcx, srcy, wmm);
     } else {
       srcx = d.SourceX[o];
       srcy = d.SourceY[o];

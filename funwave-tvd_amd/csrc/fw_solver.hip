@@ -222,7 +222,8 @@ int alloc_state(fw_ctx *c) {
   FW_CELL_FIELDS(FW_ALLOC) FW_XFACE_FIELDS(FW_ALLOC) FW_YFACE_FIELDS(FW_ALLOC)
 #undef FW_ALLOC
   void **rs[] = {(void **)&d.AGEn, (void **)&d.xa, (void **)&d.xc, (void **)&d.xd, (void **)&d.ya,
-                 (void **)&d.yc,   (void **)&d.yd, (void **)&d.c1, (void **)&d.y1, (void **)&d.y2};
+                 (void **)&d.yc,   (void **)&d.yd, (void **)&d.c1, (void **)&d.y1, (void **)&d.y2,
+                 (void **)&d.sumU, (void **)&d.sumV};
   for (void **q : rs)
     if ((rc = get(pe * sizeof(R), q)) != FW_OK) return rc;
   void **is[] = {(void **)&d.MASK, (void **)&d.MASK9, (void **)&d.MASK_STRUC, (void **)&d.MASKn, (void **)&d.cand};

@@ -127,14 +127,18 @@ int fw_reduce_stats(fw_ctx *ctx, fw_stats *out);
 /* --- options ------------------------------------------------------------ */
 /* FW_OPT_FULL_STATE = 1 keeps every intermediate array of MODULE GLOBAL up to
  * date (face states UxL..GyR, HxL.., SxL.., the wave-speed copies, the
- * first derivatives Ut, Vt, Ux.., Utx.., DUtxx.. of CAL_DISPERSION, and in
- * fw_step the source terms SourceX/Y, FrcInsX/Y, BreakSourceX/Y and
- * WaveMaker_Mass, and the mean-flow diagnostics P_center, Q_center, U_davg,
- * V_davg, Wsurf), for routine-by-routine comparison with the reference.
- * Default 0: only the arrays a later routine reads are written; results are
- * identical.  fw_source_terms always writes its outputs; the face states,
- * the dispersion first derivatives and the mean-flow diagnostics follow
- * this option in the per-routine entry points too. */
+ * first derivatives Ut, Vt, Ux.., Utx.., DUtxx.. of CAL_DISPERSION, its
+ * nonlinear terms U1pp, V1pp, U2, V2, U3, V3, and in fw_step the source
+ * terms SourceX/Y, FrcInsX/Y, BreakSourceX/Y and WaveMaker_Mass, and the
+ * mean-flow diagnostics P_center, Q_center, U_davg, V_davg, Wsurf), for
+ * routine-by-routine comparison with the reference.  Default 0: only the
+ * arrays a later routine reads are written (U1pp+U2+U3 and V1pp+V2+V3 only
+ * as the sums SourceTerms forms); results are identical.  fw_source_terms
+ * always writes its outputs; the face states, the dispersion first
+ * derivatives and nonlinear terms and the mean-flow diagnostics follow this
+ * option in the per-routine entry points too (without it, fw_source_terms
+ * takes the nonlinear terms from fw_cal_dispersion's sums, not from
+ * uploaded U1pp..V3). */
 enum { FW_OPT_FULL_STATE = 1 };
 int fw_set_option(fw_ctx *ctx, int option, int value);
 

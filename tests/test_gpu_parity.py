@@ -39,7 +39,10 @@ DISP_FIRST = {"Ut", "Vt", "Ux", "Vx", "Uy", "Vy", "DUx", "DVx", "DUy", "DVy", "U
 SOURCES = {"SourceX", "SourceY", "WaveMaker_Mass", "BreakSourceX", "BreakSourceY", "FrcInsX", "FrcInsY"}
 # and the mean-flow diagnostics of ESTIMATE_HUV, read by no routine of the step
 DIAGNOSTICS = {"P_center", "Q_center", "U_davg", "V_davg", "Wsurf"}
-PRODUCTION_SKIP = FACE_STATES | DISP_FIRST | SOURCES | DIAGNOSTICS
+# and the nonlinear dispersive terms that SourceTerms reads only as the sums
+# U1pp+U2+U3, V1pp+V2+V3 (kept as such)
+DISP_SUMMED = {"U1pp", "V1pp", "U2", "V2", "U3", "V3"}
+PRODUCTION_SKIP = FACE_STATES | DISP_FIRST | SOURCES | DIAGNOSTICS | DISP_SUMMED
 
 
 def _compare(g, o, where, full_state=True):
