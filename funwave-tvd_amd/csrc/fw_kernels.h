@@ -184,19 +184,32 @@ __global__ void k_dt_partial(Dev<R> d, Geo g, Prm<R> p, int copy_uv) {
   R t3 = Lit<R>::large();
   // workgroups stride over rows, threads over the row (a minimum does not
   // depend on the order it is taken in)
+  // four cells per pass: loads first (clamped indices, unconditional), then
+  // the copies and the minimum (a store would hold back later loads)
+  constexpr int KB = 4;
   for (int j = 1 + blockIdx.x; j <= g.N; j += gridDim.x)
-  for (int i = 1 + threadIdx.x; i <= g.M; i += blockDim.x) {
-    size_t o = IX(i, j);
-    R u = d.U[o], v = d.V[o], h = d.H[o];
-    if (copy_uv) { d.U0[o] = u; d.V0[o] = v; }
-    R c = dsqrt(GRAV * rmax(h, p.MDF));
-    R t1 = dabs(u) + c;
-    R t2 = (t1 < SMALL) ? ddiv(p.DX, SMALL) : ddiv(p.DX, t1);
-    if (t2 < t3) t3 = t2;
-    t1 = dabs(v) + c;
-    t2 = (t1 < SMALL) ? ddiv(p.DY, SMALL) : ddiv(p.DY, t1);
-    if (t2 < t3) t3 = t2;
-  }
+    for (int i0 = 1 + threadIdx.x; i0 <= g.M; i0 += KB * blockDim.x) {
+      R u[KB], v[KB], h[KB];
+#pragma unroll
+      for (int k = 0; k < KB; k++) {
+        const int i = i0 + k * blockDim.x;
+        const size_t o = IX(i <= g.M ? i : g.M, j);
+        u[k] = d.U[o]; v[k] = d.V[o]; h[k] = d.H[o];
+      }
+#pragma unroll
+      for (int k = 0; k < KB; k++) {
+        const int i = i0 + k * blockDim.x;
+        if (i > g.M) continue;
+        if (copy_uv) { d.U0[IX(i, j)] = u[k]; d.V0[IX(i, j)] = v[k]; }
+        R c = dsqrt(GRAV * rmax(h[k], p.MDF));
+        R t1 = dabs(u[k]) + c;
+        R t2 = (t1 < SMALL) ? ddiv(p.DX, SMALL) : ddiv(p.DX, t1);
+        if (t2 < t3) t3 = t2;
+        t1 = dabs(v[k]) + c;
+        t2 = (t1 < SMALL) ? ddiv(p.DY, SMALL) : ddiv(p.DY, t1);
+        if (t2 < t3) t3 = t2;
+      }
+    }
   sm[threadIdx.x] = t3;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -235,10 +248,19 @@ template <class R>
 __global__ void k_maxeta_partial(Dev<R> d, Geo g) {
   __shared__ R sm[256];
   R m = (R)0;
+  // 8 loads in flight per thread before the comparisons
   for (int j = g.Jb + blockIdx.x; j <= g.Je; j += gridDim.x)
-    for (int i = g.Ib + threadIdx.x; i <= g.Ie; i += blockDim.x) {
-      R a = dabs(d.Eta[IX(i, j)]);
-      if (a > m) m = a;
+    for (int i0 = g.Ib + threadIdx.x; i0 <= g.Ie; i0 += 8 * blockDim.x) {
+      R a[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int i = i0 + k * blockDim.x;
+        const R v = dabs(d.Eta[IX(i <= g.Ie ? i : g.Ie, j)]);  // clamped: unconditional load
+        a[k] = i <= g.Ie ? v : (R)0;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (a[k] > m) m = a[k];
     }
   sm[threadIdx.x] = m;
   __syncthreads();
