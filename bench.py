@@ -75,6 +75,8 @@ def parse():
     ap.add_argument("--eval-members", type=int, default=8)
     ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--fine-steps", type=int, default=200)
+    ap.add_argument("--workers", type=int, default=4,
+                    help="ensemble members run at a time (host threads)")
     return ap.parse_args()
 
 
@@ -99,7 +101,7 @@ def ensemble(args):
     for seed in range(6000, 6000 + args.eval_members):
         c = fwlearn.member(seed)
         evals.append(fwlearn.with_fixed_dt(c, fwlearn.stable_dt(c)))
-    r = fwlearn.compare(evals, args.fine_steps, params)
+    r = fwlearn.compare(evals, args.fine_steps, params, workers=args.workers)
     rph = r["runs_per_hour"]
     ok = r["all_learned_within_bound"]
     out = {
@@ -111,7 +113,8 @@ def ensemble(args):
         "speedup_learned_vs_fine": round(rph["learned"] / rph["fine"], 3),
         "max_err_learned": round(r["max_err_learned"], 4), "max_err_coarse": round(r["max_err_coarse"], 4),
         "config": {"workload": "conical-island solitary-wave members (INI_SOL), fixed dt, "
-                               f"{args.fine_steps} fine steps each, members run one after another",
+                               f"{args.fine_steps} fine steps each, {args.workers} members at a time "
+                               "(host threads, one context and stream pair per member)",
                    "train_members": args.train_members, "eval_members": args.eval_members,
                    "train_steps": args.train_steps, "loss_first": float(np.mean(losses[:20])),
                    "loss_last": float(np.mean(losses[-20:])),

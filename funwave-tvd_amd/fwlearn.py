@@ -212,10 +212,11 @@ def train(datasets, steps=2000, batch=16, size=64, lr=2e-3, seed=0, params=None,
     return p, losses
 
 
-def run_fine(case, steps, precision="sp"):
+def run_fine(case, steps, precision="sp", state=None):
     """Plain run: final interior (eta, MASK) and device-complete wall
-    seconds."""
-    st = fwsetup.build_state(case, precision)
+    seconds (from context creation; `state`, if given, is the case's
+    prebuilt host state)."""
+    st = state if state is not None else fwsetup.build_state(case, precision)
     t0 = time.perf_counter()
     g = fwgpu.Solver(st, precision)
     g.step(steps)
@@ -245,11 +246,11 @@ def water_surface_error(fine_eta, fine_mask, coarse_eta, coarse_mask):
     return float(np.linalg.norm(ec - ef) / max(np.linalg.norm(ef), 1e-300))
 
 
-def run_coarse(case, steps, params=None, precision="sp", start=None):
+def run_coarse(case, steps, params=None, precision="sp", start=None, state=None):
     """Coarse run of `steps` steps from `start` (restricted_start of the
     fine member) or the case's own initial state, corrected after every step
     when params is given: final interior (eta, MASK) and wall seconds."""
-    st = fwsetup.build_state(case, precision)
+    st = state if state is not None else fwsetup.build_state(case, precision)
     t0 = time.perf_counter()
     g = fwgpu.Solver(st, precision)
     if start is not None:
@@ -271,27 +272,46 @@ def run_coarse(case, steps, params=None, precision="sp", start=None):
     return (eta, mask), wall
 
 
-def compare(members, fine_steps, params, precision="sp"):
+def compare(members, fine_steps, params, precision="sp", workers=1):
     """Per member: plain fine, plain coarse and learned coarse runs over the
     same simulated time; errors against the restricted fine surface and the
-    runs/hour of each mode (members run one after another)."""
+    runs/hour of each mode.  Host states are built first; then each mode runs
+    all members, `workers` at a time from host threads (each member its own
+    context and streams), and its runs/hour is members / that batch's wall
+    time."""
+    fine_st = [fwsetup.build_state(c, precision) for c in members]
+    coarse = [coarse_case(c) for c in members]
+    coarse_st = [fwsetup.build_state(c, precision) for c in coarse]
+    starts = [restricted_start(s) for s in fine_st]
+    idx = range(len(members))
+    fwgpu.load_library()          # bind once, before any worker thread does
+    fwcnn._lib()
+
+    def batch(fn):
+        t0 = time.perf_counter()
+        if workers <= 1:
+            res = [fn(k) for k in idx]
+        else:
+            from concurrent.futures import ThreadPoolExecutor
+            with ThreadPoolExecutor(workers) as ex:
+                res = list(ex.map(fn, idx))
+        return res, time.perf_counter() - t0
+
+    fine, t_f = batch(lambda k: run_fine(members[k], fine_steps, precision, fine_st[k]))
+    plain, t_c = batch(lambda k: run_coarse(coarse[k], fine_steps // 2, None, precision, starts[k], coarse_st[k]))
+    learned, t_l = batch(lambda k: run_coarse(coarse[k], fine_steps // 2, params, precision, starts[k],
+                                              coarse_st[k]))
     rows = []
-    t_f = t_c = t_l = 0.0
-    for case in members:
-        (ef, mf), wf = run_fine(case, fine_steps, precision)
-        cc = coarse_case(case)
-        start = restricted_start(fwsetup.build_state(case, precision))
-        (ec, mc), wc = run_coarse(cc, fine_steps // 2, None, precision, start)
-        (el, ml), wl = run_coarse(cc, fine_steps // 2, params, precision, start)
+    for k, case in enumerate(members):
+        (ef, mf), wf = fine[k]
+        (ec, mc), wc = plain[k]
+        (el, ml), wl = learned[k]
         rows.append({"member": case.name, "M": case.mglob, "N": case.nglob,
                      "err_coarse": water_surface_error(ef, mf, ec, mc),
                      "err_learned": water_surface_error(ef, mf, el, ml),
                      "wall_fine": wf, "wall_coarse": wc, "wall_learned": wl})
-        t_f += wf
-        t_c += wc
-        t_l += wl
     n = len(members)
-    return {"members": rows,
+    return {"members": rows, "workers": workers,
             "runs_per_hour": {"fine": 3600.0 * n / t_f, "coarse": 3600.0 * n / t_c, "learned": 3600.0 * n / t_l},
             "all_learned_within_bound": all(r["err_learned"] <= ERR_BOUND for r in rows),
             "max_err_learned": max(r["err_learned"] for r in rows),
