@@ -73,6 +73,8 @@ __device__ __forceinline__ float ddiv(float a, float b) { return __fdiv_rn(a, b)
 __device__ __forceinline__ double ddiv(double a, double b) { return a / b; }
 // SIGN(1.0, b) with IEEE signed zero
 template <class R> __device__ __forceinline__ R sign1(R b) { return signbit(b) ? (R)-1 : (R)1; }
+__device__ __forceinline__ float copysign_(float m, float s) { return __builtin_copysignf(m, s); }
+__device__ __forceinline__ double copysign_(double m, double s) { return __builtin_copysign(m, s); }
 // x / 2 and x * 0.5 are the same correctly rounded value
 template <class R> __device__ __forceinline__ R half(R x) { return x * (R)0.5; }
 // Division by a grid spacing: a multiplication by the reciprocal when both
@@ -591,26 +593,61 @@ __device__ __forceinline__ R mm3(R a, R b, R c) {
   const R neg = rmin((R)0, rmax(rmax(a, tb), tc));
   return a >= (R)0 ? pos : neg;
 }
+// The same with the hardware minimum and maximum (v_min/v_max, fused into
+// v_min3/v_max3): for non-NaN operands they return the same number as the
+// selects of rmin/rmax, and differ at most in the sign of a zero result.
+__device__ __forceinline__ float hmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double hmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float hmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double hmax(double a, double b) { return __builtin_fmax(a, b); }
 template <class R>
-__device__ __forceinline__ R din_val(R vm2, R vm1, R v0, R vp1, int mm2, int mm1, int m0, int mp1) {
-  R t1 = vm1 - vm2, t2 = v0 - vm1, t3 = vp1 - v0;
-  R d1 = mm3(t1, t2, t3), d2 = mm3(t2, t3, t1), d3 = mm3(t3, t1, t2);
+__device__ __forceinline__ R mm3_hw(R a, R b, R c) {
+  R tb = (R)2 * b, tc = (R)2 * c;
+  const R pos = hmax((R)0, hmin(hmin(a, tb), tc));
+  const R neg = hmin((R)0, hmax(hmax(a, tb), tc));
+  return a >= (R)0 ? pos : neg;
+}
+__device__ __forceinline__ bool neg_zero(float x) { return __float_as_uint(x) == 0x80000000u; }
+__device__ __forceinline__ bool neg_zero(double x) {
+  return (unsigned long long)__double_as_longlong(x) == 0x8000000000000000ull;
+}
+// Din (CONSTRUCT_HO_X fluxes.F:1550-1593) from the slopes t1..t3 with the
+// minimum/maximum M given as a functor.
+template <class R, class MM>
+__device__ __forceinline__ R din_from(R t1, R t2, R t3, bool edge, bool dry, MM m) {
+  R d1 = m(t1, t2, t3), d2 = m(t2, t3, t1), d3 = m(t3, t1, t2);
   // next to a dry cell t1 = t3 = t2, so all three limited slopes are mm3(t2, t2, t2)
-  const R e = mm3(t2, t2, t2);
-  const bool edge = mm2 == 0 || mp1 == 0;
+  const R e = m(t2, t2, t2);
   d1 = edge ? e : d1;
   d2 = edge ? e : d2;
   d3 = edge ? e : d3;
-  const bool dry = mm1 == 0 || m0 == 0;
   d1 = dry ? (R)0 : d1;
   d2 = dry ? (R)0 : d2;
   d3 = dry ? (R)0 : d3;
   return t2 - (R)1 / (R)6 * (d3 - (R)2 * d2 + d1);
 }
+// The limited slopes d1..d3 enter Din = t2 - (1/6)(d3 - 2 d2 + d1) only
+// through that sum X.  With the hardware minimum/maximum every operation
+// sees the same numbers as with the reference's MIN/MAX, so every nonzero
+// intermediate is bitwise the same and only the sign of a zero X can
+// differ; t2 - X/6 then differs only when t2 is -0 (-0 - (+0) = -0 but
+// -0 - (-0) = +0).  Those lanes (not dry: there d1..d3 are +0 either way)
+// take the reference's selects; elsewhere Din is bitwise the same.
+// (Non-finite inputs are outside this: a NaN slope can come out different.)
+template <class R>
+__device__ __forceinline__ R din_val(R vm2, R vm1, R v0, R vp1, int mm2, int mm1, int m0, int mp1) {
+  const R t1 = vm1 - vm2, t2 = v0 - vm1, t3 = vp1 - v0;
+  const bool edge = mm2 == 0 || mp1 == 0;
+  const bool dry = mm1 == 0 || m0 == 0;
+  R D = din_from(t1, t2, t3, edge, dry, [](R a, R b, R c) { return mm3_hw(a, b, c); });
+  if (neg_zero(t2) && !dry) D = din_from(t1, t2, t3, edge, dry, [](R a, R b, R c) { return mm3(a, b, c); });
+  return D;
+}
 template <class R>
 __device__ __forceinline__ void vanleer(R a, R b, R &v1, R &v2, R &ta, R &tb) {
-  a = dabs(a) <= SMALL ? SMALL * sign1(a) : a;
-  b = dabs(b) <= SMALL ? SMALL * sign1(b) : b;
+  // SMALL * SIGN(1, a) is copysign(SMALL, a) exactly
+  a = dabs(a) <= SMALL ? copysign_(SMALL, a) : a;
+  b = dabs(b) <= SMALL ? copysign_(SMALL, b) : b;
   const R rat = ddiv(b, a);
   const R q1 = ddiv(rat + dabs(rat), (R)1 + rat);
   v1 = dabs((R)1 + rat) > SMALL ? q1 : (R)0;

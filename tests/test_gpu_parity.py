@@ -104,6 +104,44 @@ def test_fused_step_matches_oracle(name, prec, full_state, gpu_available):
     g.close()
 
 
+@pytest.mark.parametrize("prec", ["sp", "dp"])
+@pytest.mark.parametrize("name", ["tiny2d", "island"])
+def test_fluxes_with_signed_zeros(name, prec, gpu_available):
+    """FLUXES on a state where 40 % of the cells of every reconstructed field
+    hold +0 or -0 (a masked negative velocity is -0): slopes such as
+    (-0) - (+0) = -0 beside nonzero ones, where only MIN/MAX tie-breaking
+    decides the sign of a zero limited slope.  Every array bit for bit."""
+    import fwgpu
+    import oracle
+    print(f"\n[signed zeros] case={name} prec={prec}", flush=True)
+    init, steps = load_golden(name, prec)
+    st = dict(init)
+    st.update({k: v for k, v in steps[sorted(steps)[1]].items() if not np.isscalar(v)})
+    rng = np.random.default_rng(7)
+    for k in ("U", "V", "HU", "HV", "Eta"):
+        a = np.array(st[k], copy=True)
+        r = rng.random(a.shape)
+        a[r < 0.2] = 0.0
+        a[(r >= 0.2) & (r < 0.4)] = -0.0
+        st[k] = a
+    g, o = fwgpu.Solver(st, prec, full_state=True), oracle.OracleSolver(st)
+    _compare(g, o, "upload")
+    g.exchange(); o.exchange()
+    gdt, gt = g.estimate_dt(); o.estimate_dt()
+    assert (gdt, gt) == (o.dt.value, o.time.value)
+    g.save_stage0(2)
+    o["U0"][...] = o["U"]; o["V0"][...] = o["V"]
+    if init["DISPERSION"]:
+        g.dispersion(); o.dispersion()
+    g.fluxes(); o.fluxes()
+    _compare(g, o, "fluxes")
+    # the case is exercised: some slope (-0) - (+0) sits between wet cells
+    u, m = st["U"], st["MASK"]
+    t2 = u[:, 1:] - u[:, :-1]
+    assert np.any(np.signbit(t2) & (t2 == 0) & (m[:, 1:] == 1) & (m[:, :-1] == 1))
+    g.close()
+
+
 PLAN = {"tiny2d": [1, 60, 250], "tiny1d": [1, 400], "island": [1, 120], "tiny2d_dx15": [1, 150],
         "config0": [1, 200], "member3": [1, 150], "tiny2d_fsponge": [1, 120], "tiny2d_nobrk": [1, 120],
         "tiny2d_swe": [1, 120]}
