@@ -635,12 +635,25 @@ __device__ __forceinline__ R din_from(R t1, R t2, R t3, bool edge, bool dry, MM 
 // take the reference's selects; elsewhere Din is bitwise the same.
 // (Non-finite inputs are outside this: a NaN slope can come out different.)
 template <class R>
-__device__ __forceinline__ R din_val(R vm2, R vm1, R v0, R vp1, int mm2, int mm1, int m0, int mp1) {
+__device__ __forceinline__ R din_val_ref(R vm2, R vm1, R v0, R vp1, int mm2, int mm1, int m0, int mp1) {
   const R t1 = vm1 - vm2, t2 = v0 - vm1, t3 = vp1 - v0;
-  const bool edge = mm2 == 0 || mp1 == 0;
+  return din_from(t1, t2, t3, mm2 == 0 || mp1 == 0, mm1 == 0 || m0 == 0,
+                  [](R a, R b, R c) { return mm3(a, b, c); });
+}
+// Din on the hardware minimum/maximum; sets `exact` for a lane that needs
+// din_val_ref instead (t2 = -0, not dry)
+template <class R>
+__device__ __forceinline__ R din_val_hw(R vm2, R vm1, R v0, R vp1, int mm2, int mm1, int m0, int mp1, bool &exact) {
+  const R t1 = vm1 - vm2, t2 = v0 - vm1, t3 = vp1 - v0;
   const bool dry = mm1 == 0 || m0 == 0;
-  R D = din_from(t1, t2, t3, edge, dry, [](R a, R b, R c) { return mm3_hw(a, b, c); });
-  if (neg_zero(t2) && !dry) D = din_from(t1, t2, t3, edge, dry, [](R a, R b, R c) { return mm3(a, b, c); });
+  exact = exact || (neg_zero(t2) && !dry);
+  return din_from(t1, t2, t3, mm2 == 0 || mp1 == 0, dry, [](R a, R b, R c) { return mm3_hw(a, b, c); });
+}
+template <class R>
+__device__ __forceinline__ R din_val(R vm2, R vm1, R v0, R vp1, int mm2, int mm1, int m0, int mp1) {
+  bool exact = false;
+  R D = din_val_hw(vm2, vm1, v0, vp1, mm2, mm1, m0, mp1, exact);
+  if (exact) D = din_val_ref(vm2, vm1, v0, vp1, mm2, mm1, m0, mp1);
   return D;
 }
 template <class R>
@@ -752,9 +765,7 @@ __device__ __forceinline__ void pair_states(R vc, R D, R Dn, R &Lnext, R &Rc) {
 // every lane, so the loads are unconditional; lanes without dok or pok
 // discard what they form)
 template <class R>
-__device__ __forceinline__ void recon_x(const R *v, size_t o, const int (&mk)[4], bool dok, bool pok, R &L, R &Rf) {
-  const R vm2 = v[o - 2], vm1 = v[o - 1], v0 = v[o], vp1 = v[o + 1];
-  const R Dv = din_val(vm2, vm1, v0, vp1, mk[0], mk[1], mk[2], mk[3]);
+__device__ __forceinline__ void recon_x(R v0, R Dv, bool dok, bool pok, R &L, R &Rf) {
   const R D = dok ? Dv : (R)0, vc = dok ? v0 : (R)0;
   R Dn = __shfl_down(D, 1);
   R Lv, Rv;
@@ -847,18 +858,42 @@ __global__ void __launch_bounds__(256) k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
     const bool pok = c >= g.Ib - 1 && c <= g.Ie + 1;
     // (rows Jb..Je have cells on both sides of every lane's index: the
     // loads need no guard)
+    constexpr int NV = DISP ? 7 : 5;
+    const R *fv[7] = {d.U, d.V, d.HU, d.HV, d.Eta, d.U4, d.V4};
     int mk[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) mk[k] = d.MASK[o + k - 2];
-    recon_x(d.U, o, mk, dok, pok, uL, uR);
-    recon_x(d.V, o, mk, dok, pok, vL, vR);
-    recon_x(d.HU, o, mk, dok, pok, huL, huR);
-    recon_x(d.HV, o, mk, dok, pok, hvL, hvR);
-    recon_x(d.Eta, o, mk, dok, pok, eL, eR);
-    if (DISP) {
-      recon_x(d.U4, o, mk, dok, pok, u4L, u4R);
-      recon_x(d.V4, o, mk, dok, pok, v4L, v4R);
+    R Lq[NV], Rq[NV];
+    if constexpr (sizeof(R) == 8) {
+      // field by field (all fields' slopes at once cost FP64 registers)
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        const R *v = fv[q];
+        const R Dv = din_val(v[o - 2], v[o - 1], v[o], v[o + 1], mk[0], mk[1], mk[2], mk[3]);
+        recon_x(v[o], Dv, dok, pok, Lq[q], Rq[q]);
+      }
+    } else {
+      R v0[NV], Dv[NV];
+      bool exact = false;
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        const R *v = fv[q];
+        v0[q] = v[o];
+        Dv[q] = din_val_hw(v[o - 2], v[o - 1], v0[q], v[o + 1], mk[0], mk[1], mk[2], mk[3], exact);
+      }
+      if (exact) {  // rare: a -0 slope t2 in a wet cell (one branch for all fields)
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+          const R *v = fv[q];
+          Dv[q] = din_val_ref(v[o - 2], v[o - 1], v[o], v[o + 1], mk[0], mk[1], mk[2], mk[3]);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NV; q++) recon_x(v0[q], Dv[q], dok, pok, Lq[q], Rq[q]);
     }
+    uL = Lq[0]; uR = Rq[0]; vL = Lq[1]; vR = Rq[1]; huL = Lq[2]; huR = Rq[2];
+    hvL = Lq[3]; hvR = Rq[3]; eL = Lq[4]; eR = Rq[4];
+    if (DISP) { u4L = Lq[5 % NV]; u4R = Rq[5 % NV]; v4L = Lq[6 % NV]; v4R = Rq[6 % NV]; }
   }
   if (!own) return;
   if (!comp) {
