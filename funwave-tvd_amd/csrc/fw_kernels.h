@@ -1178,10 +1178,42 @@ __global__ void __launch_bounds__(256) k_flux_y_march(Dev<R> d, Geo g, Prm<R> p)
 template <class R, bool P2>
 __device__ __forceinline__ void source_terms(const Dev<R> &d, const Geo &g, const Prm<R> &p, R AA, int i, int j,
                                              bool keep, bool sums, R &sx_out, R &sy_out, R &wm_out) {
-  const size_t o = IX(i, j);
+  const size_t o = IX(i, j), oe = IX(i + 1, j), ow = IX(i - 1, j), on = IX(i, j + 1), os = IX(i, j - 1);
   const Dv<R> dx = dvx<P2>(p), dy = dvy<P2>(p);
   const R mdf = p.MDF, g1 = p.G1, g2 = p.G2;
-  R wm;
+  const bool fric = p.FRIC_SPONGE, visc = p.VISC_BRK, dsp = p.DIFF_SPONGE, brk = visc || dsp;
+  // Every load first, without branches, so that they are in flight
+  // together: a term that is switched off reads, in place of its own array,
+  // one that this cell reads at the same place anyway (a cache hit), and
+  // its value is not used.  (Every plane is allocated.)
+  const R wm_old = d.WaveMaker_Mass[o];
+  const R u = d.U[o], v = d.V[o], cd = d.Cd[o], hc = d.H[o], eta = d.Eta[o], dep = d.Depth[o];
+  const int m9i = d.MASK9[o], mki = d.MASK[o];
+  const R pe = d.P[oe], p0 = d.P[o], qn = d.Q[on], q0 = d.Q[o];
+  const R dxe = d.DepthX[oe], dx0 = d.DepthX[o], dyn = d.DepthY[on], dy0 = d.DepthY[o];
+  const R *U4 = d.U4, *V4 = d.V4, *U = d.U, *V = d.V;
+  const R u4o = U4[o], u4e = U4[oe], u4w = U4[ow], u4n = U4[on], u4s = U4[os];
+  const R v4o = V4[o], v4e = V4[oe], v4w = V4[ow], v4n = V4[on], v4s = V4[os];
+  const R ue = U[oe], uw = U[ow], un = U[on], us = U[os];
+  const R ve = V[oe], vw = V[ow], vn = V[on], vs = V[os];
+  const R u1p = d.U1p[o], v1p = d.V1p[o];
+  // (U1pp + U2) + U3 (sources.F:254): with every intermediate kept, from the
+  // MODULE GLOBAL arrays; otherwise the sums CAL_DISPERSION's kernel formed
+  const R *su[3] = {sums ? d.sumU : d.U1pp, sums ? d.sumU : d.U2, sums ? d.sumU : d.U3};
+  const R *sv[3] = {sums ? d.sumV : d.V1pp, sums ? d.sumV : d.V2, sums ? d.sumV : d.V3};
+  const R su0 = su[0][o], su1 = su[1][o], su2 = su[2][o];
+  const R sv0 = sv[0][o], sv1 = sv[1][o], sv2 = sv[2][o];
+  const R c4 = (fric ? d.CD_4_SPONGE : d.Cd)[o];
+  // viscosity at (i,j), (i+1,j), (i-1,j), (i,j+1), (i,j-1)
+  const size_t qs[5] = {o, oe, ow, on, os};
+  const R *NB = visc ? d.nu_break : U, *NS = dsp ? d.nu_sponge : U;
+  const R *HU = brk ? d.HU : U, *HV = brk ? d.HV : V;
+  R nbv[5], nsv[5], hu[5], hv[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    nbv[k] = NB[qs[k]]; nsv[k] = NS[qs[k]]; hu[k] = HU[qs[k]]; hv[k] = HV[qs[k]];
+  }
+  R wm = wm_old;
   bool keep_wm = false;  // the outputs kept for the caller are stored last
   if (p.wavemaker != 0 && i >= p.ilo && i <= p.ihi && j >= p.jlo && j <= p.jhi) {
     int bi = i - p.ilo, bj = j - p.jlo;
@@ -1196,66 +1228,55 @@ __device__ __forceinline__ void source_terms(const Dev<R> &d, const Geo &g, cons
       wm = s;
     }
     keep_wm = true;
-  } else {
-    wm = d.WaveMaker_Mass[o];
   }
-  R u = d.U[o], v = d.V[o];
   R spd = dsqrt(u * u + v * v);
-  R fix = -(d.Cd[o] * u * spd);
-  R m9r = (R)d.MASK9[o];
-  R hmax = rmax(d.H[o], mdf);
-  R divpq = dv(d.P[IX(i + 1, j)] - d.P[o], dx) + dv(d.Q[IX(i, j + 1)] - d.Q[o], dy);
-  const R *U4 = d.U4, *V4 = d.V4, *U = d.U, *V = d.V;
-  // (U1pp + U2) + U3 (sources.F:254): with every intermediate kept, from the
-  // MODULE GLOBAL arrays; otherwise the sums CAL_DISPERSION's kernel formed
-  const R dsum_u = sums ? d.sumU[o] : d.U1pp[o] + d.U2[o] + d.U3[o];
-  const R dsum_v = sums ? d.sumV[o] : d.V1pp[o] + d.V2[o] + d.V3[o];
-  R mk = (R)d.MASK[o];
-  R sx = dv(GRAV * (d.Eta[o]), dx) * (d.DepthX[IX(i + 1, j)] - d.DepthX[o]) * mk + fix
-         + g1 * m9r * hmax * (dv(u * (R)0.5 * (U4[IX(i + 1, j)] - U4[IX(i - 1, j)]), dx)
-                              + dv(v * (R)0.5 * (U4[IX(i, j + 1)] - U4[IX(i, j - 1)]), dy)
-                              + dv(U4[o] * (R)0.5 * (U[IX(i + 1, j)] - U[IX(i - 1, j)]), dx)
-                              + dv(V4[o] * (R)0.5 * (U[IX(i, j + 1)] - U[IX(i, j - 1)]), dy)
+  R fix = -(cd * u * spd);
+  R m9r = (R)m9i;
+  R hmax = rmax(hc, mdf);
+  R divpq = dv(pe - p0, dx) + dv(qn - q0, dy);
+  const R dsum_u = sums ? su0 : su0 + su1 + su2;
+  const R dsum_v = sums ? sv0 : sv0 + sv1 + sv2;
+  R mk = (R)mki;
+  R sx = dv(GRAV * (eta), dx) * (dxe - dx0) * mk + fix
+         + g1 * m9r * hmax * (dv(u * (R)0.5 * (u4e - u4w), dx)
+                              + dv(v * (R)0.5 * (u4n - u4s), dy)
+                              + dv(u4o * (R)0.5 * (ue - uw), dx)
+                              + dv(v4o * (R)0.5 * (un - us), dy)
                               - g2 * m9r * dsum_u)
-         + g1 * m9r * divpq * (U4[o] - d.U1p[o]) + wm * u;
-  R fiy = -(d.Cd[o] * v * spd);
-  R sy = dv(GRAV * (d.Eta[o]), dy) * (d.DepthY[IX(i, j + 1)] - d.DepthY[o]) * mk + fiy
-         + g1 * m9r * hmax * (dv(u * (R)0.5 * (V4[IX(i + 1, j)] - V4[IX(i - 1, j)]), dx)
-                              + dv(v * (R)0.5 * (V4[IX(i, j + 1)] - V4[IX(i, j - 1)]), dy)
-                              + dv(U4[o] * (R)0.5 * (V[IX(i + 1, j)] - V[IX(i - 1, j)]), dx)
-                              + dv(V4[o] * (R)0.5 * (V[IX(i, j + 1)] - V[IX(i, j - 1)]), dy)
+         + g1 * m9r * divpq * (u4o - u1p) + wm * u;
+  R fiy = -(cd * v * spd);
+  R sy = dv(GRAV * (eta), dy) * (dyn - dy0) * mk + fiy
+         + g1 * m9r * hmax * (dv(u * (R)0.5 * (v4e - v4w), dx)
+                              + dv(v * (R)0.5 * (v4n - v4s), dy)
+                              + dv(u4o * (R)0.5 * (ve - vw), dx)
+                              + dv(v4o * (R)0.5 * (vn - vs), dy)
                               - g2 * m9r * dsum_v)
-         + g1 * m9r * divpq * (V4[o] - d.V1p[o]) + wm * v;
-  if (p.FRIC_SPONGE) {
-    const R c4 = d.CD_4_SPONGE[o], dp = d.Depth[o];
-    sx = sx - c4 * u * spd * dp;
-    sy = sy - c4 * v * spd * dp;
+         + g1 * m9r * divpq * (v4o - v1p) + wm * v;
+  if (fric) {
+    sx = sx - c4 * u * spd * dep;
+    sy = sy - c4 * v * spd * dep;
   }
   R bx = (R)0, by = (R)0;
-  const bool brk = p.VISC_BRK || p.DIFF_SPONGE;
   if (brk) {
-    // viscosity at (i,j), (i+1,j), (i-1,j), (i,j+1), (i,j-1): nu_break, plus
-    // nu_sponge, in single precision
-    const size_t qs[5] = {o, IX(i + 1, j), IX(i - 1, j), IX(i, j + 1), IX(i, j - 1)};
+    // nu_break, plus nu_sponge, in single precision
     float nu[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (p.VISC_BRK) {
+    if (visc) {
 #pragma unroll
-      for (int k = 0; k < 5; k++) nu[k] = (float)d.nu_break[qs[k]];
+      for (int k = 0; k < 5; k++) nu[k] = (float)nbv[k];
     }
-    if (p.DIFF_SPONGE) {
+    if (dsp) {
 #pragma unroll
-      for (int k = 0; k < 5; k++) nu[k] = (float)((R)nu[k] + d.nu_sponge[qs[k]]);
+      for (int k = 0; k < 5; k++) nu[k] = (float)((R)nu[k] + nsv[k]);
     }
     float n0 = nu[0];
     R sxp = (R)(nu[1] + n0), sxm = (R)(nu[2] + n0);
     R syp = (R)(nu[3] + n0), sym = (R)(nu[4] + n0);
-    const R *HU = d.HU, *HV = d.HV;
     const R h5x = p.h5x, h5y = p.h5y;
-    bx = h5x * (dv(sxp * (R)1, dx) * (HU[IX(i + 1, j)] - HU[o]) - dv(sxm * (R)1, dx) * (HU[o] - HU[IX(i - 1, j)]))
-           + h5y * (dv(syp * (R)1, dy) * (HU[IX(i, j + 1)] - HU[o]) - dv(sym * (R)1, dy) * (HU[o] - HU[IX(i, j - 1)]));
+    bx = h5x * (dv(sxp * (R)1, dx) * (hu[1] - hu[0]) - dv(sxm * (R)1, dx) * (hu[0] - hu[2]))
+           + h5y * (dv(syp * (R)1, dy) * (hu[3] - hu[0]) - dv(sym * (R)1, dy) * (hu[0] - hu[4]));
     sx = sx + bx;
-    by = h5x * (dv(sxp * (R)1, dx) * (HV[IX(i + 1, j)] - HV[o]) - dv(sxm * (R)1, dx) * (HV[o] - HV[IX(i - 1, j)]))
-           + h5y * (dv(syp * (R)1, dy) * (HV[IX(i, j + 1)] - HV[o]) - dv(sym * (R)1, dy) * (HV[o] - HV[IX(i, j - 1)]));
+    by = h5x * (dv(sxp * (R)1, dx) * (hv[1] - hv[0]) - dv(sxm * (R)1, dx) * (hv[0] - hv[2]))
+           + h5y * (dv(syp * (R)1, dy) * (hv[3] - hv[0]) - dv(sym * (R)1, dy) * (hv[0] - hv[4]));
     sy = sy + by;
   }
   if (keep) {
@@ -1311,15 +1332,15 @@ __global__ void __launch_bounds__(WideLB<R>::wg256) k_huv(Dev<R> d, Geo g, Prm<R
     const R gx0 = d.Gx[o], gx1 = d.Gx[IX(i + 1, j)], gy0 = d.Gy[o], gy1 = d.Gy[IX(i, j + 1)];
     const R ubar0 = d.Ubar0[o], ubar = d.Ubar[o], vbar0 = d.Vbar0[o], vbar = d.Vbar[o];
     const R depth = d.Depth[o];
-    int mk = 0;
-    R m9 = (R)0, dxl = (R)0, dxr = (R)0, dyl = (R)0, dyr = (R)0, vxy = (R)0, dvxy = (R)0, uxy = (R)0, duxy = (R)0;
-    if (p.DISPERSION) {
-      mk = d.MASK[o];
-      m9 = (R)d.MASK9[o];
-      dxl = d.Depth[IX(i - 1, j)]; dxr = d.Depth[IX(i + 1, j)];
-      dyl = d.Depth[IX(i, j - 1)]; dyr = d.Depth[IX(i, j + 1)];
-      vxy = d.Vxy[o]; dvxy = d.DVxy[o]; uxy = d.Uxy[o]; duxy = d.DUxy[o];
-    }
+    // the line coefficients' inputs, also without dispersion (then they
+    // read this cell's depth and mask again, and are not used)
+    const bool disp = p.DISPERSION;
+    const int mk = d.MASK[o];
+    const R m9 = (R)(disp ? d.MASK9 : d.MASK)[o];
+    const R dxl = d.Depth[disp ? IX(i - 1, j) : o], dxr = d.Depth[disp ? IX(i + 1, j) : o];
+    const R dyl = d.Depth[disp ? IX(i, j - 1) : o], dyr = d.Depth[disp ? IX(i, j + 1) : o];
+    const R vxy = (disp ? d.Vxy : d.Depth)[o], dvxy = (disp ? d.DVxy : d.Depth)[o];
+    const R uxy = (disp ? d.Uxy : d.Depth)[o], duxy = (disp ? d.DUxy : d.Depth)[o];
     R srcx, srcy, wmm;
     if (FUSED) {
       source_terms<R, P2>(d, g, p, AA, i, j, p.full_state != 0, p.full_state == 0, srcx, srcy, wmm);
