@@ -364,8 +364,12 @@ __global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_deriv(Dev<R> d, Geo g
   // back the loads after it.
   // ETAT from the previous stage's P, Q (dispersion.F:103-119)
   // (P(i+1), Q(j+1) exist for every cell, so the loads need no branch)
+  // (ETAT is formed on each path separately: formed once ahead of the
+  // branch, its loads would be waited for there)
   const bool has_etat = i <= g.M - 1 && j <= g.N - 1;
-  const R etat = -dv(d.P[IX(i + 1, j)] - d.P[IX(i, j)], dx) - dv(d.Q[IX(i, j + 1)] - d.Q[IX(i, j)], dy);
+  auto form_etat = [&]() {
+    return -dv(d.P[IX(i + 1, j)] - d.P[IX(i, j)], dx) - dv(d.Q[IX(i, j + 1)] - d.Q[IX(i, j)], dy);
+  };
   const bool keep_t = g2 && p.full_state;  // :133-142
   R ut = (R)0, vt = (R)0;
   if (keep_t) {
@@ -373,6 +377,7 @@ __global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_deriv(Dev<R> d, Geo g
     vt = ddiv(d.V[o] - d.V0[o], DT);
   }
   if (i < g.Ib || i > g.Ie || j < g.Jb || j > g.Je) {
+    const R etat = form_etat();
     if (has_etat) d.ETAT[o] = etat;
     if (keep_t) { d.Ut[o] = ut; d.Vt[o] = vt; }
     return;
@@ -402,6 +407,7 @@ __global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_deriv(Dev<R> d, Geo g
   const bool slopes = g2 || p.SHOW_BRK;  // read by WAVE_BREAKING
   const R etax = dfx(g, d.Eta, i, j, dx) * m9;
   const R etay = dfy(g, d.Eta, i, j, dy) * m9;
+  const R etat = form_etat();
   if (has_etat) d.ETAT[o] = etat;
   if (keep_t) { d.Ut[o] = ut; d.Vt[o] = vt; }
   d.Uxx[o] = uxx; d.Uxy[o] = uxy; d.Vxy[o] = vxy; d.Vyy[o] = vyy;
