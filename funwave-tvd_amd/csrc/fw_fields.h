@@ -19,7 +19,8 @@
   X(U4) X(V4) X(U1p) X(V1p) X(U1pp) X(V1pp) X(U2) X(V2) X(U3) X(V3) \
   X(SourceX) X(SourceY) X(WaveMaker_Mass) X(P_center) X(Q_center) \
   X(U_davg) X(V_davg) X(Wsurf) X(BreakSourceX) X(BreakSourceY) \
-  X(FrcInsX) X(FrcInsY) X(CD_4_SPONGE) X(nu_sponge)
+  X(FrcInsX) X(FrcInsY) X(CD_4_SPONGE) X(nu_sponge) \
+  X(ROLLER_FLUX) X(UNDERTOW_U) X(UNDERTOW_V) X(ETAmean)
 #define FW_XFACE_FIELDS(X) \
   X(DepthX) X(P) X(Fx) X(Gx) X(UxL) X(UxR) X(VxL) X(VxR) X(HUxL) X(HUxR) \
   X(HVxL) X(HVxR) X(EtaRxL) X(EtaRxR) X(HxL) X(HxR) X(U4xL) X(U4xR) \

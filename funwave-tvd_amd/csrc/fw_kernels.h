@@ -1860,7 +1860,7 @@ __global__ void k_mask(Dev<R> d, Geo g, Prm<R> p) {
 // which is the in-order result because the dependencies form a DAG.
 template <class R>
 struct BrkIn {
-  R t1, t2, t3, ang, et, c, px, py, pxy;
+  R t1, t2, t3, ang, et, c, px, py, pxy, t4, cg;  // cg = SQRT(GRAV*ABS(H))
 };
 template <class R>
 __device__ __forceinline__ BrkIn<R> brk_in(const Dev<R> &d, const Geo &g, const Prm<R> &p, size_t o) {
@@ -1872,8 +1872,9 @@ __device__ __forceinline__ BrkIn<R> brk_in(const Dev<R> &d, const Geo &g, const 
   R ex = d.ETAx[o], ey = d.ETAy[o];
   b.ang = ddiv(datan2(-ey, -ex) * (R)180, FW_PI);
   b.et = d.ETAT[o];
-  R t4 = rmax(dsqrt(ex * ex + ey * ey), SMALL);
-  b.c = rmin(ddiv(dabs(b.et), t4), dsqrt(GRAV * dabs(h)));
+  b.t4 = rmax(dsqrt(ex * ex + ey * ey), SMALL);
+  b.cg = dsqrt(GRAV * dabs(h));
+  b.c = rmin(ddiv(dabs(b.et), b.t4), b.cg);
   b.pxy = ddiv(dsqrt(p.DX * p.DX + p.DY * p.DY), rmax(b.c, SMALL));
   b.px = ddiv(dsqrt(p.DX * p.DX), rmax(b.c, SMALL));
   b.py = ddiv(dsqrt(p.DY * p.DY), rmax(b.c, SMALL));
@@ -1904,9 +1905,12 @@ __device__ __forceinline__ R brk_prop(const Dev<R> &d, const Geo &g, const BrkIn
   if (q == 4) t = brk_trig(old[IX(i, j + 1)], b.py, old[IX(i - 1, j + 1)], b.pxy, nw[IX(i - 1, j)], b.px, DT);
   return t ? DT : own;
 }
+// nu_break of a cell with its final age; `roll` is set in continuing
+// breaking outside the wavemaker band, where the roller is formed too
 template <class R>
 __device__ __forceinline__ R brk_nu(const Dev<R> &d, const Geo &g, const Prm<R> &p, const BrkIn<R> &b, size_t o, int i,
-                                    int j, R age) {
+                                    int j, R age, bool &roll) {
+  roll = false;
   R xmk = (R)(i - g.Ib) * p.DX, ymk = (R)(j - g.Jb) * p.DY;
   if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < half(p.YW)) {
     if (b.et > rmin(b.t2, p.WKC * b.t3)) {
@@ -1917,9 +1921,24 @@ __device__ __forceinline__ R brk_nu(const Dev<R> &d, const Geo &g, const Prm<R> 
   }
   if (age > (R)0 && age < p.TBRK && b.et > b.t2) {
     R cap1 = (R)1 * (rmax(d.Depth[o], p.MDF) + d.Eta[o]);
+    roll = true;
     return cap1 * b.t2 + p.NUB;
   }
   return (R)0 + p.NUB;
+}
+// Roller flux and undertow of continuing breaking (breaker.F:326-337);
+// elsewhere the arrays keep their values.  0.45 is a default-REAL literal.
+template <class R>
+__device__ __forceinline__ void brk_roller(const Dev<R> &d, const BrkIn<R> &b, size_t o) {
+  const R u = d.U[o], v = d.V[o], eta = d.Eta[o], em = d.ETAmean[o];
+  const R c1 = rmax(ddiv(dabs(b.et), b.t4), b.cg);
+  R r = dabs(ddiv((R)0.45f * b.et, rmax(c1, SMALL)));
+  r = rmin(r, (R)(sizeof(R) == 4 ? (R)0.1638f : (R)0.1638));
+  const R rf = dabs(c1 - dsqrt(u * u + v * v)) * r * (eta - em);
+  const R th = b.ang * FW_PI / (R)180;
+  d.ROLLER_FLUX[o] = rf;
+  d.UNDERTOW_U[o] = -(rf * dcos(th));
+  d.UNDERTOW_V[o] = -(rf * dsin(th));
 }
 
 template <class R>
@@ -1936,6 +1955,7 @@ __global__ void k_brk1(Dev<R> d, Geo g, Prm<R> p, R DT) {
   }
   BrkIn<R> b = brk_in(d, g, p, o);
   R na;
+  bool cand = false;  // final age (and roller) settled by k_brk2
   if (b.et >= b.t1 && (age == (R)0 || age > p.TBRK)) {
     na = DT;
   } else if (age > (R)0) {
@@ -1949,12 +1969,15 @@ __global__ void k_brk1(Dev<R> d, Geo g, Prm<R> p, R DT) {
         if (q != 3) {
           int k = atomicAdd(d.cand_n, 1);
           d.cand[k] = (int)o;
+          cand = true;
         }
       }
     }
   }
   d.AGEn[o] = na;
-  d.nu_break[o] = brk_nu(d, g, p, b, o, i, j, na);
+  bool roll;
+  d.nu_break[o] = brk_nu(d, g, p, b, o, i, j, na, roll);
+  if (roll && !cand) brk_roller(d, b, o);
 }
 
 template <class R>
@@ -1991,7 +2014,9 @@ __global__ void __launch_bounds__(1024) k_brk2(Dev<R> d, Geo g, Prm<R> p, R DT) 
     size_t o = (size_t)d.cand[k];
     int i = 1 + (int)(o % g.P), j = 1 + (int)(o / g.P);
     BrkIn<R> b = brk_in(d, g, p, o);
-    d.nu_break[o] = brk_nu(d, g, p, b, o, i, j, d.AGEn[o]);
+    bool roll;
+    d.nu_break[o] = brk_nu(d, g, p, b, o, i, j, d.AGEn[o], roll);
+    if (roll) brk_roller(d, b, o);
   }
   __syncthreads();
   if (threadIdx.x == 0) *d.cand_n = 0;

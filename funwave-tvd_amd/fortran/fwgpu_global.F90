@@ -116,6 +116,8 @@ CONTAINS
     CALL put_r('WaveMaker_Mass', nc, WaveMaker_Mass)
     IF (ALLOCATED(CD_4_SPONGE)) CALL put_r('CD_4_SPONGE', nc, CD_4_SPONGE)
     IF (ALLOCATED(nu_sponge)) CALL put_r('nu_sponge', nc, nu_sponge)
+    CALL put_r('ROLLER_FLUX', nc, ROLLER_FLUX); CALL put_r('UNDERTOW_U', nc, UNDERTOW_U)
+    CALL put_r('UNDERTOW_V', nc, UNDERTOW_V); CALL put_r('ETAmean', nc, ETAmean)
     CALL put_r('DepthX', nx, DepthX); CALL put_r('P', nx, P); CALL put_r('Fx', nx, Fx); CALL put_r('Gx', nx, Gx)
     CALL put_r('DepthY', ny, DepthY); CALL put_r('Q', ny, Q); CALL put_r('Fy', ny, Fy); CALL put_r('Gy', ny, Gy)
     IF (ALLOCATED(xmk_wk)) CALL put_r('xmk_wk', Mloc, xmk_wk)
@@ -138,6 +140,8 @@ CONTAINS
     CALL get_r('AGE_BREAKING', nc, AGE_BREAKING); CALL get_r('nu_break', nc, nu_break)
     CALL get_r('ETAT', nc, ETAT); CALL get_r('ETAx', nc, ETAx); CALL get_r('ETAy', nc, ETAy)
     CALL get_r('SourceX', nc, SourceX); CALL get_r('SourceY', nc, SourceY)
+    CALL get_r('ROLLER_FLUX', nc, ROLLER_FLUX); CALL get_r('UNDERTOW_U', nc, UNDERTOW_U)
+    CALL get_r('UNDERTOW_V', nc, UNDERTOW_V)
     CALL get_r('DepthX', nx, DepthX); CALL get_r('P', nx, P); CALL get_r('Fx', nx, Fx); CALL get_r('Gx', nx, Gx)
     CALL get_r('DepthY', ny, DepthY); CALL get_r('Q', ny, Q); CALL get_r('Fy', ny, Fy); CALL get_r('Gy', ny, Gy)
     CALL get_r('SxL', nx, SxL); CALL get_r('SxR', nx, SxR); CALL get_r('SyL', ny, SyL); CALL get_r('SyR', ny, SyR)

@@ -1079,7 +1079,7 @@ void fwo_update_mask(const fwo_params *p, fwo_fields *f) {
 
 /* ------------------------------------------------------------------ */
 /* WAVE_BREAKING -> BREAKING, breaker.F:60-353 (default eddy-viscosity
- * branch; ROLLER/UNDERTOW diagnostics are not part of the state) */
+ * branch, with the roller flux and undertow of continuing breaking) */
 void fwo_breaking(const fwo_params *p, fwo_fields *f, double dt_) {
   DIMS;
   if (!p->SHOW_BREAKING) return;
@@ -1132,6 +1132,15 @@ void fwo_breaking(const fwo_params *p, fwo_fields *f, double dt_) {
         if (C_(AGE, i, j) > ZERO && C_(AGE, i, j) < tbrk && et > t2) {
           R cap1 = FLC(1.0) * (rmax(C_(F(Depth), i, j), mdf) + C_(F(Eta), i, j));
           C_(F(nu_break), i, j) = cap1 * t2 + nub;
+          /* roller and undertow, breaker.F:326-337 (kept elsewhere) */
+          R cr = rmax(RABS(et) / t4, RSQRT(GRAV * RABS(C_(H, i, j))));
+          R rr = RABS(FLC(0.45) * et / rmax(cr, SMALL));
+          rr = rmin(rr, SPC(0.1638));
+          R uu = C_(F(U), i, j), vv = C_(F(V), i, j);
+          R rf = RABS(cr - RSQRT(uu * uu + vv * vv)) * rr * (C_(F(Eta), i, j) - C_(F(ETAmean), i, j));
+          C_(F(ROLLER_FLUX), i, j) = rf;
+          C_(F(UNDERTOW_U), i, j) = -(rf * RCOS(ang * PI / SPC(180.0)));
+          C_(F(UNDERTOW_V), i, j) = -(rf * RSIN(ang * PI / SPC(180.0)));
         } else {
           C_(F(nu_break), i, j) = ZERO + nub;
         }

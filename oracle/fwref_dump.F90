@@ -160,6 +160,8 @@ CONTAINS
     CALL dr2('FrcInsX', FrcInsX); CALL dr2('FrcInsY', FrcInsY)
     IF (ALLOCATED(CD_4_SPONGE)) CALL dr2('CD_4_SPONGE', CD_4_SPONGE)
     IF (ALLOCATED(nu_sponge)) CALL dr2('nu_sponge', nu_sponge)
+    CALL dr2('ROLLER_FLUX', ROLLER_FLUX); CALL dr2('UNDERTOW_U', UNDERTOW_U)
+    CALL dr2('UNDERTOW_V', UNDERTOW_V); CALL dr2('ETAmean', ETAmean)
     ! x-faces (Mloc+1, Nloc)
     CALL dr2('DepthX', DepthX); CALL dr2('P', P); CALL dr2('Fx', Fx); CALL dr2('Gx', Gx)
     CALL dr2('UxL', UxL); CALL dr2('UxR', UxR); CALL dr2('VxL', VxL); CALL dr2('VxR', VxR)

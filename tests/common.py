@@ -58,3 +58,30 @@ def rel_l2(a, b):
     d = np.linalg.norm(a - b)
     n = np.linalg.norm(b)
     return float(d / n) if n > 0 else float(d)
+
+
+def load_breaking_golden(prec):
+    """(state WAVE_BREAKING reads, arrays it writes) at step 250, stage 3 of
+    tiny2d, from tests/golden/breaking_tiny2d_<prec>.npz."""
+    z = np.load(os.path.join(GOLDEN, f"breaking_tiny2d_{prec}.npz"), allow_pickle=False)
+    pre, post = {}, {}
+    for k in z.files:
+        grp, key = k.split("/", 1)
+        v = z[k]
+        if v.ndim == 0:
+            v = v.item()
+        (pre if grp == "pre" else post)[key] = v
+    return pre, post
+
+
+def ulp_distance(a, b):
+    """Largest distance in units in the last place between two arrays of
+    the same float dtype (0 for identical bits)."""
+    a, b = np.asarray(a), np.asarray(b)
+    it = np.int64 if a.dtype == np.float64 else np.int32
+    ia, ib = a.view(it).astype(np.int64), b.view(it).astype(np.int64)
+    # map the sign-magnitude bit patterns onto a monotone integer line
+    top = np.int64(1) << (63 if it is np.int64 else 31)
+    ia = np.where(ia < 0, -(ia + top) if it is np.int64 else -(ia & 0x7fffffff), ia)
+    ib = np.where(ib < 0, -(ib + top) if it is np.int64 else -(ib & 0x7fffffff), ib)
+    return int(np.max(np.abs(ia - ib))) if a.size else 0
