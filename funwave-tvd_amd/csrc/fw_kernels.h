@@ -662,6 +662,42 @@ __device__ __forceinline__ R din_val(R vm2, R vm1, R v0, R vp1, int mm2, int mm1
   if (exact) D = din_val_ref(vm2, vm1, v0, vp1, mm2, mm1, m0, mp1);
   return D;
 }
+// Quotient a/b by Markstein's method (FP32): y = RN(1/b) from v_rcp_f32 and
+// one Newton step, q = RN(a*y), r = a - b*q exactly (fma), q' = RN(q + r*y),
+// with q's sign for a zero a.  Given y = RN(1/b), q' is the correctly rounded
+// quotient when nothing under- or overflows; checked against the division
+// bit for bit on gfx950 (the reciprocal for every significand of every
+// exponent in -62..62, quotients on 1.6e9 operand pairs with exponents in
+// -62..62) and, on the CPU from y = RN(1/b), exhaustively over the
+// significand of a for divisors with an all-ones significand.  Callers keep
+// their operands within 2^-62..2^62 and fall back to ddiv otherwise.
+__device__ __forceinline__ float mdiv(float a, float b) {
+  const float y0 = __builtin_amdgcn_rcpf(b);
+  const float y = __builtin_fmaf(__builtin_fmaf(-b, y0, 1.0f), y0, y0);
+  const float q = a * y;
+  const float q1 = __builtin_fmaf(__builtin_fmaf(-b, q, a), y, q);
+  return __builtin_copysignf(q1, q);
+}
+__device__ __forceinline__ double mdiv(double a, double b) { return a / b; }
+// The van Leer weights with mdiv: with |a|, |b| in [SMALL, 2^30] (SMALL is
+// about 2^-19.9) the ratios lie in 2^-50..2^50 and every numerator and
+// denominator in 2^-50..2^51 (or is 0, or a denominator 1 + rat is 0: the
+// quotient is then discarded as in the reference), inside mdiv's range.
+// `exact` is set for a lane outside it, which must use vanleer instead.
+template <class R>
+__device__ __forceinline__ void vanleer_m(R a, R b, R &v1, R &v2, R &ta, R &tb, bool &exact) {
+  a = dabs(a) <= SMALL ? copysign_(SMALL, a) : a;
+  b = dabs(b) <= SMALL ? copysign_(SMALL, b) : b;
+  exact = exact || !(dabs(a) <= (R)1073741824 && dabs(b) <= (R)1073741824);
+  const R rat = mdiv(b, a);
+  const R q1 = mdiv(rat + dabs(rat), (R)1 + rat);
+  v1 = dabs((R)1 + rat) > SMALL ? q1 : (R)0;
+  const R rat2 = mdiv(a, b);
+  const R q2 = mdiv(rat2 + dabs(rat2), (R)1 + rat2);
+  v2 = dabs((R)1 + rat2) > SMALL ? q2 : (R)0;
+  ta = a;
+  tb = b;
+}
 template <class R>
 __device__ __forceinline__ void vanleer(R a, R b, R &v1, R &v2, R &ta, R &tb) {
   // SMALL * SIGN(1, a) is copysign(SMALL, a) exactly
@@ -764,18 +800,27 @@ __device__ __forceinline__ void pair_states(R vc, R D, R Dn, R &Lnext, R &Rc) {
   Lnext = vc + ((R)1 / (R)6) * (w1 * a + (R)2 * w2 * b);
   Rc = vc - ((R)1 / (R)6) * ((R)2 * w1 * a + w2 * b);
 }
+// the same with vanleer_m (sets `exact` where pair_states must be used)
+template <class R>
+__device__ __forceinline__ void pair_states_m(R vc, R D, R Dn, R &Lnext, R &Rc, bool &exact) {
+  R w1, w2, a, b;
+  vanleer_m(D, Dn, w1, w2, a, b, exact);
+  Lnext = vc + ((R)1 / (R)6) * (w1 * a + (R)2 * w2 * b);
+  Rc = vc - ((R)1 / (R)6) * ((R)2 * w1 * a + w2 * b);
+}
 
 // left/right states of face c along x for field v: lanes with dok hold a
 // valid D(c), lanes with pok a valid pair for cell c
 // (o is an in-row index with two cells to its left and one to its right for
 // every lane, so the loads are unconditional; lanes without dok or pok
 // discard what they form)
-template <class R>
-__device__ __forceinline__ void recon_x(R v0, R Dv, bool dok, bool pok, R &L, R &Rf) {
+template <class R, bool M = false>
+__device__ __forceinline__ void recon_x(R v0, R Dv, bool dok, bool pok, R &L, R &Rf, bool *exact = nullptr) {
   const R D = dok ? Dv : (R)0, vc = dok ? v0 : (R)0;
   R Dn = __shfl_down(D, 1);
   R Lv, Rv;
-  pair_states(vc, D, Dn, Lv, Rv);
+  if constexpr (M) pair_states_m(vc, D, Dn, Lv, Rv, *exact);
+  else pair_states(vc, D, Dn, Lv, Rv);
   const R Ln = pok ? Lv : (R)0, Rc = pok ? Rv : (R)0;
   L = __shfl_up(Ln, 1);
   Rf = Rc;
@@ -894,8 +939,16 @@ __global__ void __launch_bounds__(256) k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
           Dv[q] = din_val_ref(v[o - 2], v[o - 1], v[o], v[o + 1], mk[0], mk[1], mk[2], mk[3]);
         }
       }
+      // pairs with mdiv; if any lane of the wave has a slope beyond its range,
+      // the whole wave forms them again with the division (the neighbours'
+      // states come through shuffles)
+      bool mx = false;
 #pragma unroll
-      for (int q = 0; q < NV; q++) recon_x(v0[q], Dv[q], dok, pok, Lq[q], Rq[q]);
+      for (int q = 0; q < NV; q++) recon_x<R, true>(v0[q], Dv[q], dok, pok, Lq[q], Rq[q], &mx);
+      if (__any(mx)) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) recon_x(v0[q], Dv[q], dok, pok, Lq[q], Rq[q]);
+      }
     }
     uL = Lq[0]; uR = Rq[0]; vL = Lq[1]; vR = Rq[1]; huL = Lq[2]; huR = Rq[2];
     hvL = Lq[3]; hvR = Rq[3]; eL = Lq[4]; eR = Rq[4];
@@ -1140,12 +1193,22 @@ __global__ void __launch_bounds__(256) k_flux_y_march(Dev<R> d, Geo g, Prm<R> p)
       const int m = c - 2;
       const bool pok = col && m >= g.Jb - 1 && m <= g.Je + 1;
       R Rc[NV], Ln[NV];
+      bool mx = false;  // a slope beyond mdiv's range: the division instead
 #pragma unroll
       for (int q = 0; q < NV; q++) {
         R lv, rv;
-        pair_states(w[1][q], Dp[q], Dn[q], lv, rv);
+        pair_states_m(w[1][q], Dp[q], Dn[q], lv, rv, mx);
         Ln[q] = pok ? lv : (R)0;
         Rc[q] = pok ? rv : (R)0;
+      }
+      if (mx) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+          R lv, rv;
+          pair_states(w[1][q], Dp[q], Dn[q], lv, rv);
+          Ln[q] = pok ? lv : (R)0;
+          Rc[q] = pok ? rv : (R)0;
+        }
       }
       if (t >= 5 && m <= N1) {
         const bool comp = col && m >= g.Jb && m <= g.Je + 1;
