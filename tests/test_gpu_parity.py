@@ -142,6 +142,36 @@ def test_fluxes_with_signed_zeros(name, prec, gpu_available):
     g.close()
 
 
+@pytest.mark.parametrize("prec", ["sp", "dp"])
+def test_fluxes_with_slopes_beyond_quotient_range(prec, gpu_available):
+    """FLUXES where 2 % of the cells of HU and HV hold +-1e10, so that some
+    limited slopes exceed 2^30: the FP32 van Leer weights then take the
+    correctly rounded division instead of the remainder-corrected quotient.
+    Every array bit for bit against the oracle."""
+    import fwgpu
+    import oracle
+    print(f"\n[large slopes] prec={prec}", flush=True)
+    init, steps = load_golden("tiny2d", prec)
+    st = dict(init)
+    st.update({k: v for k, v in steps[sorted(steps)[1]].items() if not np.isscalar(v)})
+    rng = np.random.default_rng(5)
+    for k in ("HU", "HV"):
+        a = np.array(st[k], copy=True)
+        sel = rng.random(a.shape) < 0.02
+        a[sel] = np.where(rng.random(a.shape) < 0.5, 1e10, -1e10)[sel].astype(a.dtype)
+        st[k] = a
+    g, o = fwgpu.Solver(st, prec, full_state=True), oracle.OracleSolver(st)
+    g.exchange(); o.exchange()
+    gdt, gt = g.estimate_dt(); o.estimate_dt()
+    assert (gdt, gt) == (o.dt.value, o.time.value)
+    g.save_stage0(2)
+    o["U0"][...] = o["U"]; o["V0"][...] = o["V"]
+    g.fluxes(); o.fluxes()
+    _compare(g, o, "fluxes")
+    assert np.nanmax(np.abs(o["HUxL"])) > 2.0 ** 30  # the large slopes reach the states
+    g.close()
+
+
 PLAN = {"tiny2d": [1, 60, 250], "tiny1d": [1, 400], "island": [1, 120], "tiny2d_dx15": [1, 150],
         "config0": [1, 200], "member3": [1, 150], "tiny2d_fsponge": [1, 120], "tiny2d_nobrk": [1, 120],
         "tiny2d_swe": [1, 120]}
