@@ -1143,8 +1143,10 @@ __global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo 
 // row loads); no LDS, no barrier; the strip's two end slopes and one pair
 // are formed again by the neighbouring strip.  Same operations on the same
 // operands as the cell-by-cell reconstruction.
+// (five waves per SIMD: 96 VGPRs with a few spilled dwords beat four waves
+// at 103)
 template <class R, bool DISP, int RS>
-__global__ void __launch_bounds__(256) k_flux_y_march(Dev<R> d, Geo g, Prm<R> p) {
+__global__ void __launch_bounds__(256, 5) k_flux_y_march(Dev<R> d, Geo g, Prm<R> p) {
   constexpr int NV = DISP ? 7 : 5;
   const int i = 1 + blockIdx.x * 64 + threadIdx.x;
   const int jb = (blockIdx.y * blockDim.y + threadIdx.y) * RS;

@@ -451,7 +451,7 @@ int do_fluxes(fw_ctx *c) {
   else
     launch(c, "flux_x", k_flux_x<R, false>, gx, kB2, d, g, c->prm<R>());
   if constexpr (sizeof(R) == 4) {
-    // FP32: columns marched in strips of 4 faces (125 VGPRs, independent waves)
+    // FP32: columns marched in strips of 4 faces (96 VGPRs, independent waves)
     constexpr int RS = 4;
     const int nt = (g.N + 1 + RS - 1) / RS;
     const dim3 by(64, 4), gy((g.M + 63) / 64, (nt + 3) / 4);
