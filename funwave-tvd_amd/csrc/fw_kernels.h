@@ -69,6 +69,15 @@ __device__ __forceinline__ float dcos(float x) { return (float)cos((double)x); }
 __device__ __forceinline__ double dcos(double x) { return cos(x); }
 __device__ __forceinline__ float dsin(float x) { return (float)sin((double)x); }
 __device__ __forceinline__ double dsin(double x) { return sin(x); }
+// SIN and COS of one argument with one range reduction (the same values as
+// dsin, dcos)
+__device__ __forceinline__ void dsincos(float x, float &s, float &c) {
+  double sd, cd;
+  sincos((double)x, &sd, &cd);
+  s = (float)sd;
+  c = (float)cd;
+}
+__device__ __forceinline__ void dsincos(double x, double &s, double &c) { sincos(x, &s, &c); }
 __device__ __forceinline__ float ddiv(float a, float b) { return __fdiv_rn(a, b); }
 __device__ __forceinline__ double ddiv(double a, double b) { return a / b; }
 // SIGN(1.0, b) with IEEE signed zero
@@ -1925,7 +1934,7 @@ __global__ void k_mask(Dev<R> d, Geo g, Prm<R> p) {
 // which is the in-order result because the dependencies form a DAG.
 template <class R>
 struct BrkIn {
-  R t1, t2, t3, ang, et, c, px, py, pxy, t4, cg;  // cg = SQRT(GRAV*ABS(H))
+  R t1, t2, t3, ang, et, c, px, py, pxy;
 };
 template <class R>
 __device__ __forceinline__ BrkIn<R> brk_in(const Dev<R> &d, const Geo &g, const Prm<R> &p, size_t o) {
@@ -1937,9 +1946,8 @@ __device__ __forceinline__ BrkIn<R> brk_in(const Dev<R> &d, const Geo &g, const 
   R ex = d.ETAx[o], ey = d.ETAy[o];
   b.ang = ddiv(datan2(-ey, -ex) * (R)180, FW_PI);
   b.et = d.ETAT[o];
-  b.t4 = rmax(dsqrt(ex * ex + ey * ey), SMALL);
-  b.cg = dsqrt(GRAV * dabs(h));
-  b.c = rmin(ddiv(dabs(b.et), b.t4), b.cg);
+  R t4 = rmax(dsqrt(ex * ex + ey * ey), SMALL);
+  b.c = rmin(ddiv(dabs(b.et), t4), dsqrt(GRAV * dabs(h)));
   b.pxy = ddiv(dsqrt(p.DX * p.DX + p.DY * p.DY), rmax(b.c, SMALL));
   b.px = ddiv(dsqrt(p.DX * p.DX), rmax(b.c, SMALL));
   b.py = ddiv(dsqrt(p.DY * p.DY), rmax(b.c, SMALL));
@@ -1970,12 +1978,9 @@ __device__ __forceinline__ R brk_prop(const Dev<R> &d, const Geo &g, const BrkIn
   if (q == 4) t = brk_trig(old[IX(i, j + 1)], b.py, old[IX(i - 1, j + 1)], b.pxy, nw[IX(i - 1, j)], b.px, DT);
   return t ? DT : own;
 }
-// nu_break of a cell with its final age; `roll` is set in continuing
-// breaking outside the wavemaker band, where the roller is formed too
 template <class R>
 __device__ __forceinline__ R brk_nu(const Dev<R> &d, const Geo &g, const Prm<R> &p, const BrkIn<R> &b, size_t o, int i,
-                                    int j, R age, bool &roll) {
-  roll = false;
+                                    int j, R age) {
   R xmk = (R)(i - g.Ib) * p.DX, ymk = (R)(j - g.Jb) * p.DY;
   if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < half(p.YW)) {
     if (b.et > rmin(b.t2, p.WKC * b.t3)) {
@@ -1986,24 +1991,44 @@ __device__ __forceinline__ R brk_nu(const Dev<R> &d, const Geo &g, const Prm<R> 
   }
   if (age > (R)0 && age < p.TBRK && b.et > b.t2) {
     R cap1 = (R)1 * (rmax(d.Depth[o], p.MDF) + d.Eta[o]);
-    roll = true;
     return cap1 * b.t2 + p.NUB;
   }
   return (R)0 + p.NUB;
 }
-// Roller flux and undertow of continuing breaking (breaker.F:326-337);
-// elsewhere the arrays keep their values.  0.45 is a default-REAL literal.
+
+// Roller flux and undertow of continuing breaking (breaker.F:326-337): the
+// cells outside the wavemaker band whose final age is in (0, T_brk) with
+// ETAt above Cbrk2*sqrt(g max(H, MinDepthFrc)) (the branch that sets
+// nu_break there); elsewhere the arrays keep their values.  0.45 is a
+// default-REAL literal.  They read U and V as WAVE_BREAKING finds them, after
+// the line solves, which run beside WAVE_BREAKING on the other stream, so
+// they are a kernel of their own, after both; output diagnostics, read by no
+// routine of the step, formed only when every intermediate is kept.
 template <class R>
-__device__ __forceinline__ void brk_roller(const Dev<R> &d, const BrkIn<R> &b, size_t o) {
+__global__ void k_roller(Dev<R> d, Geo g, Prm<R> p) {
+  const Tile tb = xcd_tile();
+  const int i = 1 + tb.x * blockDim.x + threadIdx.x;
+  const int j = 1 + tb.y * blockDim.y + threadIdx.y;
+  if (i < g.Ib - 1 || i > g.Ie + 1 || j < g.Jb - 1 || j > g.Je + 1) return;
+  const R xmk = (R)(i - g.Ib) * p.DX, ymk = (R)(j - g.Jb) * p.DY;
+  if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < half(p.YW)) return;
+  const size_t o = IX(i, j);
+  const R age = d.AGE_BREAKING[o], et = d.ETAT[o], h = d.H[o];
+  const R t2 = p.C2 * dsqrt(GRAV * rmax(p.MDF, h));
+  if (!(age > (R)0 && age < p.TBRK && et > t2)) return;
   const R u = d.U[o], v = d.V[o], eta = d.Eta[o], em = d.ETAmean[o];
-  const R c1 = rmax(ddiv(dabs(b.et), b.t4), b.cg);
-  R r = dabs(ddiv((R)0.45f * b.et, rmax(c1, SMALL)));
+  const R ex = d.ETAx[o], ey = d.ETAy[o];
+  const R ang = ddiv(datan2(-ey, -ex) * (R)180, FW_PI);
+  const R t4 = rmax(dsqrt(ex * ex + ey * ey), SMALL);
+  const R c1 = rmax(ddiv(dabs(et), t4), dsqrt(GRAV * dabs(h)));
+  R r = dabs(ddiv((R)0.45f * et, rmax(c1, SMALL)));
   r = rmin(r, (R)(sizeof(R) == 4 ? (R)0.1638f : (R)0.1638));
   const R rf = dabs(c1 - dsqrt(u * u + v * v)) * r * (eta - em);
-  const R th = b.ang * FW_PI / (R)180;
+  R sn, cs;
+  dsincos(ang * FW_PI / (R)180, sn, cs);
   d.ROLLER_FLUX[o] = rf;
-  d.UNDERTOW_U[o] = -(rf * dcos(th));
-  d.UNDERTOW_V[o] = -(rf * dsin(th));
+  d.UNDERTOW_U[o] = -(rf * cs);
+  d.UNDERTOW_V[o] = -(rf * sn);
 }
 
 template <class R>
@@ -2020,7 +2045,6 @@ __global__ void k_brk1(Dev<R> d, Geo g, Prm<R> p, R DT) {
   }
   BrkIn<R> b = brk_in(d, g, p, o);
   R na;
-  bool cand = false;  // final age (and roller) settled by k_brk2
   if (b.et >= b.t1 && (age == (R)0 || age > p.TBRK)) {
     na = DT;
   } else if (age > (R)0) {
@@ -2034,15 +2058,12 @@ __global__ void k_brk1(Dev<R> d, Geo g, Prm<R> p, R DT) {
         if (q != 3) {
           int k = atomicAdd(d.cand_n, 1);
           d.cand[k] = (int)o;
-          cand = true;
         }
       }
     }
   }
   d.AGEn[o] = na;
-  bool roll;
-  d.nu_break[o] = brk_nu(d, g, p, b, o, i, j, na, roll);
-  if (roll && !cand) brk_roller(d, b, o);
+  d.nu_break[o] = brk_nu(d, g, p, b, o, i, j, na);
 }
 
 template <class R>
@@ -2079,9 +2100,7 @@ __global__ void __launch_bounds__(1024) k_brk2(Dev<R> d, Geo g, Prm<R> p, R DT) 
     size_t o = (size_t)d.cand[k];
     int i = 1 + (int)(o % g.P), j = 1 + (int)(o / g.P);
     BrkIn<R> b = brk_in(d, g, p, o);
-    bool roll;
-    d.nu_break[o] = brk_nu(d, g, p, b, o, i, j, d.AGEn[o], roll);
-    if (roll) brk_roller(d, b, o);
+    d.nu_break[o] = brk_nu(d, g, p, b, o, i, j, d.AGEn[o]);
   }
   __syncthreads();
   if (threadIdx.x == 0) *d.cand_n = 0;

@@ -507,6 +507,8 @@ template <class R>
 int do_mask(fw_ctx *c);
 template <class R>
 int do_breaking(fw_ctx *c);
+template <class R>
+int do_roller(fw_ctx *c);
 
 // overlap: UPDATE_MASK and WAVE_BREAKING follow on the second stream,
 // concurrently with the line solves
@@ -543,7 +545,7 @@ int do_src_huv(fw_ctx *c, int istage, bool overlap) {
   if (rc) return rc;
   HIPCHK(c, hipEventRecord(c->ev_join, c->side));
   HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
-  return FW_OK;
+  return do_roller<R>(c);
 }
 
 // line solves and the final velocities (GET_Eta_U_V_HU_HV after the
@@ -589,6 +591,16 @@ int do_breaking(fw_ctx *c) {
   launch(c, "brk1", k_brk1<R>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), (R)c->dt);
   launch(c, "brk2", k_brk2<R>, dim3(1), dim3(1024), d, g, c->prm<R>(), (R)c->dt);
   std::swap(d.AGE_BREAKING, d.AGEn);
+  return FW_OK;
+}
+
+// roller flux and undertow (with every intermediate kept), after
+// WAVE_BREAKING and after the line solves
+template <class R>
+int do_roller(fw_ctx *c) {
+  if (!c->cfg.SHOW_BREAKING || !c->prm<R>().full_state) return FW_OK;
+  const Geo &g = c->g;
+  launch(c, "roller", k_roller<R>, grid2(g.M, g.N, kB2), kB2, c->dev<R>(), g, c->prm<R>());
   return FW_OK;
 }
 
@@ -924,6 +936,7 @@ int fw_update_mask(fw_ctx *c) { ENTRY(c, DISPATCH(c, do_mask)(c)); }
 int fw_wave_breaking(fw_ctx *c) {
   if (!c) return FW_E_INVALID;
   int rc = DISPATCH(c, do_breaking)(c);
+  if (rc == FW_OK) rc = DISPATCH(c, do_roller)(c);
   if (rc) return rc;
   int flag = 0;
   void *fl = c->rb == 8 ? (void *)c->dd.flags : (void *)c->df.flags;

@@ -11,6 +11,7 @@ MODULE FWGPU_BIND
   INTEGER(C_INT), PARAMETER :: FW_OK = 0, FW_E_INVALID = -1, FW_E_HIP = -2, &
        FW_E_NOMEM = -3, FW_E_BLOWUP = -4, FW_E_NOTFOUND = -5, FW_E_BREAK_ITER = -6
   INTEGER(C_INT), PARAMETER :: FW_WK_NONE = 0, FW_WK_REG = 1, FW_WK_DATA2D = 2
+  INTEGER(C_INT), PARAMETER :: FW_OPT_FULL_STATE = 1
 
   TYPE, BIND(C) :: fw_config
     INTEGER(C_INT) :: real_bytes
@@ -121,6 +122,11 @@ MODULE FWGPU_BIND
     INTEGER(C_INT) FUNCTION fw_synchronize(ctx) BIND(C, NAME='fw_synchronize')
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION fw_set_option(ctx, option, value) BIND(C, NAME='fw_set_option')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: option, value
     END FUNCTION
   END INTERFACE
 

@@ -96,6 +96,10 @@ CONTAINS
     c%D_gen = D_gen; c%Beta_gen = Beta_gen; c%rlamda = rlamda
     c%Tperiod = Tperiod; c%Time_ramp = Time_ramp; c%FreqPeak = FreqPeak
     CALL fwgpu_check(fw_create(c, fwctx), 'fw_create')
+    ! the roller flux and undertow are kept up to date only with every
+    ! intermediate array (output diagnostics, read by no routine of the step)
+    IF (OUT_ROLLER .OR. OUT_UNDERTOW) &
+      CALL fwgpu_check(fw_set_option(fwctx, FW_OPT_FULL_STATE, 1_C_INT), 'fw_set_option')
     CALL FWGPU_SYNC_DEVICE
   END SUBROUTINE FWGPU_INIT
 

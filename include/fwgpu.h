@@ -129,8 +129,9 @@ int fw_reduce_stats(fw_ctx *ctx, fw_stats *out);
  * date (face states UxL..GyR, HxL.., SxL.., the wave-speed copies, the
  * first derivatives Ut, Vt, Ux.., Utx.., DUtxx.. of CAL_DISPERSION, its
  * nonlinear terms U1pp, V1pp, U2, V2, U3, V3, and in fw_step the source
- * terms SourceX/Y, FrcInsX/Y, BreakSourceX/Y and WaveMaker_Mass, and the
- * mean-flow diagnostics P_center, Q_center, U_davg, V_davg, Wsurf), for
+ * terms SourceX/Y, FrcInsX/Y, BreakSourceX/Y and WaveMaker_Mass, the
+ * mean-flow diagnostics P_center, Q_center, U_davg, V_davg, Wsurf, and the
+ * roller flux and undertow ROLLER_FLUX, UNDERTOW_U, UNDERTOW_V), for
  * routine-by-routine comparison with the reference.  Default 0: only the
  * arrays a later routine reads are written (U1pp+U2+U3 and V1pp+V2+V3 only
  * as the sums SourceTerms forms); results are identical.  fw_source_terms

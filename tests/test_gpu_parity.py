@@ -37,8 +37,9 @@ DISP_FIRST = {"Ut", "Vt", "Ux", "Vx", "Uy", "Vy", "DUx", "DVx", "DUy", "DVy", "U
               "Utxx", "Utxy", "Vtxy", "Vtyy", "DUtx", "DVty", "DUtxx", "DUtxy", "DVtxy", "DVtyy"}
 # and the source terms, formed inside the RK update of their own cell
 SOURCES = {"SourceX", "SourceY", "WaveMaker_Mass", "BreakSourceX", "BreakSourceY", "FrcInsX", "FrcInsY"}
-# and the mean-flow diagnostics of ESTIMATE_HUV, read by no routine of the step
-DIAGNOSTICS = {"P_center", "Q_center", "U_davg", "V_davg", "Wsurf"}
+# and the mean-flow diagnostics of ESTIMATE_HUV and the roller flux and
+# undertow of WAVE_BREAKING, read by no routine of the step
+DIAGNOSTICS = {"P_center", "Q_center", "U_davg", "V_davg", "Wsurf", "ROLLER_FLUX", "UNDERTOW_U", "UNDERTOW_V"}
 # and the nonlinear dispersive terms that SourceTerms reads only as the sums
 # U1pp+U2+U3, V1pp+V2+V3 (kept as such)
 DISP_SUMMED = {"U1pp", "V1pp", "U2", "V2", "U3", "V3"}
