@@ -111,10 +111,13 @@ def ensemble(args):
         "accepted": ok, "plain_fine_runs_per_hour": round(rph["fine"], 1),
         "learned_runs_per_hour": round(rph["learned"], 1), "plain_coarse_runs_per_hour": round(rph["coarse"], 1),
         "speedup_learned_vs_fine": round(rph["learned"] / rph["fine"], 3),
+        "fallbacks_to_fine": r["fallbacks"],
         "max_err_learned": round(r["max_err_learned"], 4), "max_err_coarse": round(r["max_err_coarse"], 4),
         "config": {"workload": "conical-island solitary-wave members (INI_SOL), fixed dt, "
                                f"{args.fine_steps} fine steps each, {args.workers} members at a time "
-                               "(host threads, one context and stream pair per member)",
+                               "(host threads, one context and stream pair per member); a learned member "
+                               f"whose water volume drifts by more than {fwlearn.FALLBACK_DRIFT:g} falls back to "
+                               "its fine run (counted in the learned time)",
                    "train_members": args.train_members, "eval_members": args.eval_members,
                    "train_steps": args.train_steps, "loss_first": float(np.mean(losses[:20])),
                    "loss_last": float(np.mean(losses[-20:])),

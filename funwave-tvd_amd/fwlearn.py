@@ -246,10 +246,18 @@ def water_surface_error(fine_eta, fine_mask, coarse_eta, coarse_mask):
     return float(np.linalg.norm(ec - ef) / max(np.linalg.norm(ef), 1e-300))
 
 
+def water_volume(eta, wet, base):
+    """Water volume of interior cells, sum over wet cells of (eta + depth)
+    dx dy (conserved by the scheme away from sponges and sources)."""
+    d = interior(base["Depth"]).astype(np.float64)
+    return float(np.sum(np.where(wet, eta + d, 0.0))) * float(base["DX"]) * float(base["DY"])
+
+
 def run_coarse(case, steps, params=None, precision="sp", start=None, state=None):
     """Coarse run of `steps` steps from `start` (restricted_start of the
     fine member) or the case's own initial state, corrected after every step
-    when params is given: final interior (eta, MASK) and wall seconds."""
+    when params is given: final interior (eta, MASK), wall seconds and the
+    water volume at the end (water_volume)."""
     st = state if state is not None else fwsetup.build_state(case, precision)
     t0 = time.perf_counter()
     g = fwgpu.Solver(st, precision)
@@ -269,16 +277,21 @@ def run_coarse(case, steps, params=None, precision="sp", start=None, state=None)
     if net is not None:
         net.close()
     g.close()
-    return (eta, mask), wall
+    return (eta, mask), wall, water_volume(eta, mask > 0, st)
 
 
-def compare(members, fine_steps, params, precision="sp", workers=1):
+FALLBACK_DRIFT = 1e-4
+
+
+def compare(members, fine_steps, params, precision="sp", workers=1, fallback_drift=FALLBACK_DRIFT):
     """Per member: plain fine, plain coarse and learned coarse runs over the
     same simulated time; errors against the restricted fine surface and the
     runs/hour of each mode.  Host states are built first; then each mode runs
     all members, `workers` at a time from host threads (each member its own
     context and streams), and its runs/hour is members / that batch's wall
-    time."""
+    time.  A learned member whose mass drift exceeds `fallback_drift` (of
+    its water volume) falls back to the fine run: its result is the fine
+    one, and the learned mode's time includes a batch of those fine runs."""
     fine_st = [fwsetup.build_state(c, precision) for c in members]
     coarse = [coarse_case(c) for c in members]
     coarse_st = [fwsetup.build_state(c, precision) for c in coarse]
@@ -287,32 +300,41 @@ def compare(members, fine_steps, params, precision="sp", workers=1):
     fwgpu.load_library()          # bind once, before any worker thread does
     fwcnn._lib()
 
-    def batch(fn):
+    def batch(fn, which=idx):
         t0 = time.perf_counter()
         if workers <= 1:
-            res = [fn(k) for k in idx]
+            res = [fn(k) for k in which]
         else:
             from concurrent.futures import ThreadPoolExecutor
             with ThreadPoolExecutor(workers) as ex:
-                res = list(ex.map(fn, idx))
+                res = list(ex.map(fn, which))
         return res, time.perf_counter() - t0
 
     fine, t_f = batch(lambda k: run_fine(members[k], fine_steps, precision, fine_st[k]))
     plain, t_c = batch(lambda k: run_coarse(coarse[k], fine_steps // 2, None, precision, starts[k], coarse_st[k]))
     learned, t_l = batch(lambda k: run_coarse(coarse[k], fine_steps // 2, params, precision, starts[k],
                                               coarse_st[k]))
+    # water volume of each coarse start; drift = |V(end) - V(start)| / V(start)
+    vol = [water_volume(starts[k][0], starts[k][0] > -interior(coarse_st[k]["Depth"]), coarse_st[k]) for k in idx]
+    drift = lambda res, k: abs(res[k][2] - vol[k]) / vol[k]  # noqa: E731
+    fb = [k for k in idx if not (drift(learned, k) <= fallback_drift)]
+    t_fb = batch(lambda k: run_fine(members[k], fine_steps, precision, fine_st[k]), fb)[1] if fb else 0.0
     rows = []
     for k, case in enumerate(members):
         (ef, mf), wf = fine[k]
-        (ec, mc), wc = plain[k]
-        (el, ml), wl = learned[k]
+        (ec, mc), wc, _ = plain[k]
+        (el, ml), wl, _ = learned[k]
+        back = k in fb
         rows.append({"member": case.name, "M": case.mglob, "N": case.nglob,
                      "err_coarse": water_surface_error(ef, mf, ec, mc),
-                     "err_learned": water_surface_error(ef, mf, el, ml),
+                     "err_learned": 0.0 if back else water_surface_error(ef, mf, el, ml),
+                     "err_learned_uncorrected_fallback": water_surface_error(ef, mf, el, ml),
+                     "drift_coarse": drift(plain, k), "drift_learned": drift(learned, k), "fallback": back,
                      "wall_fine": wf, "wall_coarse": wc, "wall_learned": wl})
     n = len(members)
-    return {"members": rows, "workers": workers,
-            "runs_per_hour": {"fine": 3600.0 * n / t_f, "coarse": 3600.0 * n / t_c, "learned": 3600.0 * n / t_l},
+    return {"members": rows, "workers": workers, "fallbacks": len(fb),
+            "runs_per_hour": {"fine": 3600.0 * n / t_f, "coarse": 3600.0 * n / t_c,
+                              "learned": 3600.0 * n / (t_l + t_fb)},
             "all_learned_within_bound": all(r["err_learned"] <= ERR_BOUND for r in rows),
             "max_err_learned": max(r["err_learned"] for r in rows),
             "max_err_coarse": max(r["err_coarse"] for r in rows)}
