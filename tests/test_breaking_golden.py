@@ -66,3 +66,33 @@ def test_gpu_roller_uses_mean_surface(prec, gpu_available):
     for k in ("UNDERTOW_U", "UNDERTOW_V"):
         assert ulp_distance(g.download(k), o[k]) <= 2, k
     g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not oracle_available(), reason="oracle not built")
+@pytest.mark.parametrize("prec", ["sp", "dp"])
+def test_fused_step_roller_matches_oracle(prec, gpu_available):
+    """fw_step with every intermediate kept, from tiny2d's step-250 state
+    (breaking under way), 5 steps: the roller flux and undertow against the
+    oracle's whole steps.  In fw_step the breaking kernels run beside the
+    line solves; the roller must read U and V after them.  Same cells, and
+    within the whole-run tolerance (1e-5 / 1e-10 relative L2: the steps
+    evaluate ATAN2/COS/SIN per cell)."""
+    import fwgpu
+    import oracle
+    from common import load_golden, rel_l2
+    print(f"\n[fused roller] prec={prec}", flush=True)
+    init, steps = load_golden("tiny2d", prec)
+    st = dict(init)
+    st.update({k: v for k, v in steps[250].items() if not np.isscalar(v)})
+    st["DT"], st["TIME"] = steps[250]["DT"], steps[250]["TIME"]
+    g, o = fwgpu.Solver(st, prec, full_state=True), oracle.OracleSolver(st)
+    g.step(5)
+    o.run(5)
+    tol = 1e-5 if prec == "sp" else 1e-10
+    assert np.count_nonzero(o["ROLLER_FLUX"]) > 0
+    for k in ("ROLLER_FLUX", "UNDERTOW_U", "UNDERTOW_V"):
+        a, b = g.download(k), o[k]
+        assert np.array_equal(a != 0, b != 0), k
+        assert rel_l2(a, b) <= tol, (k, rel_l2(a, b))
+    g.close()
