@@ -115,9 +115,11 @@ def ensemble(args):
         "max_err_learned": round(r["max_err_learned"], 4), "max_err_coarse": round(r["max_err_coarse"], 4),
         "config": {"workload": "conical-island solitary-wave members (INI_SOL), fixed dt, "
                                f"{args.fine_steps} fine steps each, {args.workers} members at a time "
-                               "(host threads, one context and stream pair per member); a learned member "
-                               f"whose water volume drifts by more than {fwlearn.FALLBACK_DRIFT:g} falls back to "
-                               "its fine run (counted in the learned time)",
+                               "(host threads, one context and stream pair per member); the correction "
+                               "applied without its wet-cell mean of eta (moves no water); a learned member "
+                               f"whose water volume drifts by more than {fwlearn.FALLBACK_DRIFT:g} or whose "
+                               f"energy grows by more than x{fwlearn.FALLBACK_ENERGY:g} falls back to its fine run "
+                               "(counted in the learned time)",
                    "train_members": args.train_members, "eval_members": args.eval_members,
                    "train_steps": args.train_steps, "loss_first": float(np.mean(losses[:20])),
                    "loss_last": float(np.mean(losses[-20:])),

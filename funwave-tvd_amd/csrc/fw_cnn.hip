@@ -378,17 +378,44 @@ __global__ void k_pack(const float *__restrict__ p, Packs k) {
   if (t < 16) k.bias4[t] = t < kCout ? p[oB4 + t] : 0.f;
 }
 
-// correction of the coarse state from the network output (planar [3][H][W])
+// wet-cell mean of the eta correction (R)(o0*s0), accumulated in FP64 by
+// one workgroup in a fixed order (deterministic); 0 without wet cells
+template <class R>
+__global__ void __launch_bounds__(1024) k_wet_mean(const float *__restrict__ o, const int *mask, int P, int Ib, int Jb,
+                                                   int H, int W, float s0, double *mean) {
+  __shared__ double ss[1024], sc[1024];
+  const int t = threadIdx.x;
+  double s = 0, c = 0;
+  for (int k = t; k < H * W; k += 1024) {
+    const int y = k / W, x = k - y * W;
+    if (mask[(size_t)(Jb - 1 + y) * P + (Ib - 1 + x)] >= 1) {
+      s += (double)(R)(o[k] * s0);
+      c += 1;
+    }
+  }
+  ss[t] = s;
+  sc[t] = c;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w) { ss[t] += ss[t + w]; sc[t] += sc[t + w]; }
+    __syncthreads();
+  }
+  if (t == 0) mean[0] = sc[0] > 0 ? ss[0] / sc[0] : 0.0;
+}
+
+// correction of the coarse state from the network output (planar [3][H][W]);
+// with mean, the wet-cell mean of the eta correction is taken out of it
 template <class R>
 __global__ void k_correct(const float *__restrict__ o, R *eta, R *u, R *v, R *h, R *hu, R *hv, R *ubar, R *vbar,
                           const R *dep, const int *mask, int P, int Ib, int Jb, int H, int W, float s0, float s1,
-                          float s2, R g3, R mdf) {
+                          float s2, R g3, R mdf, const double *mean) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
   if (x >= W || y >= H) return;
   const size_t q = (size_t)(Jb - 1 + y) * P + (Ib - 1 + x);
   if (mask[q] < 1) return;
   const size_t k = (size_t)y * W + x, hw = (size_t)H * W;
-  const R de = (R)(o[k] * s0), du = (R)(o[hw + k] * s1), dv = (R)(o[2 * hw + k] * s2);
+  const R de = (R)(o[k] * s0) - (mean ? (R)mean[0] : (R)0);
+  const R du = (R)(o[hw + k] * s1), dv = (R)(o[2 * hw + k] * s2);
   const R e = eta[q] + de;
   const R hh = e * g3 + dep[q];
   const R hm = hh > mdf ? hh : mdf;
@@ -418,6 +445,8 @@ struct fw_cnn {
   bf16 *a0 = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr;
   bf16 *g4 = nullptr, *g3 = nullptr, *g2 = nullptr, *g1 = nullptr;
   float *in = nullptr, *out = nullptr, *tgt = nullptr;
+  double *mean = nullptr;  // wet-cell mean of the eta correction
+  int conserve = 0;
   int adam_t = 0;
   std::string err;
 };
@@ -496,6 +525,7 @@ int fw_cnn_create(int H, int W, int max_batch, fw_cnn **out) {
       (rc = cget(n, 9 * 512 * 2, &n->pk.b2)) || (rc = cget(n, 9 * 512 * 2, &n->pk.b3)) ||
       (rc = cget(n, 3 * 512 * 2, &n->pk.b4)) || (rc = cget(n, 32, &n->pk.bias1)) ||
       (rc = cget(n, 32, &n->pk.bias2)) || (rc = cget(n, 32, &n->pk.bias3)) || (rc = cget(n, 16, &n->pk.bias4)) ||
+      (rc = cget(n, 1, &n->mean)) ||
       (rc = cget(n, px * 8, &n->a0)) || (rc = cget(n, px * kC, &n->a1)) || (rc = cget(n, px * kC, &n->a2)) ||
       (rc = cget(n, px * kC, &n->a3)) || (rc = cget(n, px * 8, &n->g4)) || (rc = cget(n, px * kC, &n->g3)) ||
       (rc = cget(n, px * kC, &n->g2)) || (rc = cget(n, px * kC, &n->g1)) ||
@@ -625,6 +655,12 @@ int fw_cnn_time_forward(fw_cnn *n, int B, int iters, double *ms) {
   return FW_OK;
 }
 
+int fw_cnn_set_conserve(fw_cnn *n, int on) {
+  if (!n) return FW_E_INVALID;
+  n->conserve = on != 0;
+  return FW_OK;
+}
+
 int fw_cnn_correct(fw_cnn *n, fw_ctx *solver, const float fs[4], const float os[3]) {
   if (!n || !solver || !fs || !os) return FW_E_INVALID;
   fw_config cfg;
@@ -659,14 +695,22 @@ int fw_cnn_correct(fw_cnn *n, fw_ctx *solver, const float fs[4], const float os[
   n->stream = s;
   forward(n, 1);
   n->stream = keep;
+  const double *mean = n->conserve ? n->mean : nullptr;
   if (cfg.real_bytes == 8) {
+    if (mean)
+      hipLaunchKernelGGL(k_wet_mean<double>, dim3(1), dim3(1024), 0, s, n->out, (const int *)mask, P, cfg.Ibeg,
+                         cfg.Jbeg, Hs, Ws, os[0], n->mean);
     hipLaunchKernelGGL(k_correct<double>, g2, b2, 0, s, n->out, (double *)eta, (double *)u, (double *)v, (double *)h,
                        (double *)hu, (double *)hv, (double *)ub, (double *)vb, (const double *)dep, (const int *)mask,
-                       P, cfg.Ibeg, cfg.Jbeg, Hs, Ws, os[0], os[1], os[2], cfg.Gamma3, cfg.MinDepthFrc);
+                       P, cfg.Ibeg, cfg.Jbeg, Hs, Ws, os[0], os[1], os[2], cfg.Gamma3, cfg.MinDepthFrc, mean);
   } else {
+    if (mean)
+      hipLaunchKernelGGL(k_wet_mean<float>, dim3(1), dim3(1024), 0, s, n->out, (const int *)mask, P, cfg.Ibeg,
+                         cfg.Jbeg, Hs, Ws, os[0], n->mean);
     hipLaunchKernelGGL(k_correct<float>, g2, b2, 0, s, n->out, (float *)eta, (float *)u, (float *)v, (float *)h,
                        (float *)hu, (float *)hv, (float *)ub, (float *)vb, (const float *)dep, (const int *)mask, P,
-                       cfg.Ibeg, cfg.Jbeg, Hs, Ws, os[0], os[1], os[2], (float)cfg.Gamma3, (float)cfg.MinDepthFrc);
+                       cfg.Ibeg, cfg.Jbeg, Hs, Ws, os[0], os[1], os[2], (float)cfg.Gamma3, (float)cfg.MinDepthFrc,
+                       mean);
   }
   CHK(n, hipGetLastError());
   return FW_OK;

@@ -18,7 +18,7 @@ NPARAMS = 9 * 5 * 32 + 32 + 2 * (9 * 32 * 32 + 32) + 9 * 32 * 3 + 3
 SHAPES = [("W1", (9, 5, 32)), ("b1", (32,)), ("W2", (9, 32, 32)), ("b2", (32,)),
           ("W3", (9, 32, 32)), ("b3", (32,)), ("W4", (9, 32, 3)), ("b4", (3,))]
 EXPORTS = ["fw_cnn_create", "fw_cnn_destroy", "fw_cnn_strerror", "fw_cnn_set_params",
-           "fw_cnn_get_params", "fw_cnn_forward", "fw_cnn_train_step", "fw_cnn_correct",
+           "fw_cnn_get_params", "fw_cnn_forward", "fw_cnn_train_step", "fw_cnn_correct", "fw_cnn_set_conserve",
            "fw_cnn_time_forward"]
 # multiply-adds x 2 per cell of the four layers (algorithmic, unpadded channels)
 FLOP_PER_CELL = 2 * 9 * (5 * 32 + 32 * 32 + 32 * 32 + 32 * 3)
@@ -74,6 +74,7 @@ def _lib():
         lib.fw_cnn_forward.argtypes = [vp, fp, fp, ip]
         lib.fw_cnn_train_step.argtypes = [vp, fp, fp, ip, ct.c_float, fp, fp]
         lib.fw_cnn_correct.argtypes = [vp, vp, fp, fp]
+        lib.fw_cnn_set_conserve.argtypes = [vp, ip]
         lib.fw_cnn_time_forward.argtypes = [vp, ip, ip, ct.POINTER(ct.c_double)]
         lib._fwcnn_typed = True
     return lib
@@ -144,6 +145,11 @@ class Net:
         ms = ct.c_double()
         self._check(self.lib.fw_cnn_time_forward(self.ptr, B, iters, ct.byref(ms)))
         return ms.value
+
+    def set_conserve(self, on: bool = True):
+        """Take the wet-cell mean of the eta correction out of it (no net
+        change of water volume)."""
+        self._check(self.lib.fw_cnn_set_conserve(self.ptr, 1 if on else 0))
 
     def correct(self, solver: "fwgpu.Solver", fs=FEATURE_SCALE, os_=OUTPUT_SCALE):
         """Queue the in-place correction of a coarse solver's state."""

@@ -253,19 +253,23 @@ def water_volume(eta, wet, base):
     return float(np.sum(np.where(wet, eta + d, 0.0))) * float(base["DX"]) * float(base["DY"])
 
 
-def run_coarse(case, steps, params=None, precision="sp", start=None, state=None):
+def run_coarse(case, steps, params=None, precision="sp", start=None, state=None, conserve=False):
     """Coarse run of `steps` steps from `start` (restricted_start of the
     fine member) or the case's own initial state, corrected after every step
-    when params is given: final interior (eta, MASK), wall seconds and the
-    water volume at the end (water_volume)."""
+    when params is given (with conserve, a correction that moves no water
+    volume: fwcnn.Net.set_conserve): final interior (eta, MASK), wall seconds
+    the water volume at the end (water_volume) and the ratio of the
+    energy at the end to that at the start (fw_reduce_stats)."""
     st = state if state is not None else fwsetup.build_state(case, precision)
     t0 = time.perf_counter()
     g = fwgpu.Solver(st, precision)
     if start is not None:
         set_state(g, st, *start)
+    e_start = g.stats()["energy"]
     net = None
     if params is not None:
         net = fwcnn.Net(case.nglob, case.mglob, 1, params)
+        net.set_conserve(conserve)
     for _ in range(steps):
         g.step(1)
         if net is not None:
@@ -274,24 +278,31 @@ def run_coarse(case, steps, params=None, precision="sp", start=None, state=None)
     wall = time.perf_counter() - t0
     eta = interior(g.download("Eta")).astype(np.float64)
     mask = interior(g.download("MASK")).astype(np.float64)
+    e_end = g.stats()["energy"]
     if net is not None:
         net.close()
     g.close()
-    return (eta, mask), wall, water_volume(eta, mask > 0, st)
+    return (eta, mask), wall, water_volume(eta, mask > 0, st), e_end / e_start if e_start > 0 else float("inf")
 
 
 FALLBACK_DRIFT = 1e-4
+FALLBACK_ENERGY = 1.1
 
 
-def compare(members, fine_steps, params, precision="sp", workers=1, fallback_drift=FALLBACK_DRIFT):
+def compare(members, fine_steps, params, precision="sp", workers=1, fallback_drift=FALLBACK_DRIFT,
+            fallback_energy=FALLBACK_ENERGY, conserve=True):
     """Per member: plain fine, plain coarse and learned coarse runs over the
     same simulated time; errors against the restricted fine surface and the
     runs/hour of each mode.  Host states are built first; then each mode runs
     all members, `workers` at a time from host threads (each member its own
     context and streams), and its runs/hour is members / that batch's wall
-    time.  A learned member whose mass drift exceeds `fallback_drift` (of
-    its water volume) falls back to the fine run: its result is the fine
-    one, and the learned mode's time includes a batch of those fine runs."""
+    time.  A learned member whose water volume drifts by more than
+    `fallback_drift` of itself, or whose energy grows by more than the
+    factor `fallback_energy` (the scheme only dissipates it), falls back to
+    the fine run: its result is the fine one, and the learned mode's time
+    includes a batch of those fine runs.
+    With conserve the learned runs apply the correction without its
+    wet-cell mean of eta (run_coarse)."""
     fine_st = [fwsetup.build_state(c, precision) for c in members]
     coarse = [coarse_case(c) for c in members]
     coarse_st = [fwsetup.build_state(c, precision) for c in coarse]
@@ -313,23 +324,24 @@ def compare(members, fine_steps, params, precision="sp", workers=1, fallback_dri
     fine, t_f = batch(lambda k: run_fine(members[k], fine_steps, precision, fine_st[k]))
     plain, t_c = batch(lambda k: run_coarse(coarse[k], fine_steps // 2, None, precision, starts[k], coarse_st[k]))
     learned, t_l = batch(lambda k: run_coarse(coarse[k], fine_steps // 2, params, precision, starts[k],
-                                              coarse_st[k]))
+                                              coarse_st[k], conserve))
     # water volume of each coarse start; drift = |V(end) - V(start)| / V(start)
     vol = [water_volume(starts[k][0], starts[k][0] > -interior(coarse_st[k]["Depth"]), coarse_st[k]) for k in idx]
     drift = lambda res, k: abs(res[k][2] - vol[k]) / vol[k]  # noqa: E731
-    fb = [k for k in idx if not (drift(learned, k) <= fallback_drift)]
+    fb = [k for k in idx if not (drift(learned, k) <= fallback_drift and learned[k][3] <= fallback_energy)]
     t_fb = batch(lambda k: run_fine(members[k], fine_steps, precision, fine_st[k]), fb)[1] if fb else 0.0
     rows = []
     for k, case in enumerate(members):
         (ef, mf), wf = fine[k]
-        (ec, mc), wc, _ = plain[k]
-        (el, ml), wl, _ = learned[k]
+        (ec, mc), wc, _, gc = plain[k]
+        (el, ml), wl, _, gl = learned[k]
         back = k in fb
         rows.append({"member": case.name, "M": case.mglob, "N": case.nglob,
                      "err_coarse": water_surface_error(ef, mf, ec, mc),
                      "err_learned": 0.0 if back else water_surface_error(ef, mf, el, ml),
                      "err_learned_uncorrected_fallback": water_surface_error(ef, mf, el, ml),
-                     "drift_coarse": drift(plain, k), "drift_learned": drift(learned, k), "fallback": back,
+                     "drift_coarse": drift(plain, k), "drift_learned": drift(learned, k),
+                     "energy_ratio_coarse": gc, "energy_ratio_learned": gl, "fallback": back,
                      "wall_fine": wf, "wall_coarse": wc, "wall_learned": wl})
     n = len(members)
     return {"members": rows, "workers": workers, "fallbacks": len(fb),

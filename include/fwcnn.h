@@ -64,6 +64,11 @@ int fw_cnn_time_forward(fw_cnn *net, int B, int iters, double *ms);
  * stream. */
 int fw_cnn_correct(fw_cnn *net, fw_ctx *solver, const float fs[4], const float os[3]);
 
+/* With on != 0, fw_cnn_correct takes the wet-cell mean of the eta
+ * correction out of it (accumulated in FP64), so that the correction moves
+ * no water volume; u and v are corrected as they are.  Default off. */
+int fw_cnn_set_conserve(fw_cnn *net, int on);
+
 #ifdef __cplusplus
 }
 #endif

@@ -183,3 +183,43 @@ def test_correction_updates_wet_cells(gpu_available):
     np.testing.assert_array_equal(after["Eta"][:3], before["Eta"][:3])
     net.close()
     g.close()
+
+
+@pytest.mark.gpu
+def test_conserving_correction_moves_no_water(gpu_available):
+    """With set_conserve the wet-cell mean of the eta correction is taken
+    out: a constant network output leaves eta unchanged (u, v still move),
+    and a random network changes the wet-cell sum of eta only by rounding."""
+    import fwcnn
+    import fwgpu
+    from common import load_golden
+    print("\n[conserving correction] tiny2d sp", flush=True)
+    init, _ = load_golden("tiny2d", "sp")
+    g = fwgpu.Solver(init, "sp")
+    g.step(2)
+    H, W = init["Nloc"] - 6, init["Mloc"] - 6
+    inner = (slice(3, -3), slice(3, -3))
+    wet = g.download("MASK")[inner] == 1
+    os_ = np.array([1e-3, 2e-3, 4e-3], np.float32)
+    d = fwcnn.unflatten(np.zeros(fwcnn.NPARAMS, np.float32))
+    d["b4"][...] = [2.0, -1.0, 0.5]
+    net = fwcnn.Net(H, W, 1, fwcnn.flatten(d))
+    net.set_conserve(True)
+    e0, u0 = g.download("Eta")[inner], g.download("U")[inner]
+    net.correct(g, os_=os_)
+    g.synchronize()
+    e1, u1 = g.download("Eta")[inner], g.download("U")[inner]
+    np.testing.assert_array_equal(e1, e0)
+    assert np.all(u1[wet] != u0[wet])
+    net.close()
+    net = fwcnn.Net(H, W, 1, fwcnn.init_params(seed=3, last_scale=1.0))
+    net.set_conserve(True)
+    e0 = g.download("Eta")[inner].astype(np.float64)
+    net.correct(g, os_=os_)
+    g.synchronize()
+    e1 = g.download("Eta")[inner].astype(np.float64)
+    moved = np.sum(np.abs(e1 - e0)[wet])
+    assert moved > 0
+    assert abs(np.sum((e1 - e0)[wet])) <= 1e-3 * moved
+    net.close()
+    g.close()
