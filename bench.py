@@ -75,7 +75,7 @@ def parse():
     ap.add_argument("--eval-members", type=int, default=8)
     ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--fine-steps", type=int, default=200)
-    ap.add_argument("--workers", type=int, default=4,
+    ap.add_argument("--workers", type=int, default=8,
                     help="ensemble members run at a time (host threads)")
     return ap.parse_args()
 
