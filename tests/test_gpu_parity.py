@@ -173,6 +173,27 @@ def test_fluxes_with_slopes_beyond_quotient_range(prec, gpu_available):
     g.close()
 
 
+def test_full_size_beach_step_matches_oracle(gpu_available):
+    """The benchmark workload itself (2048 x 2048 beach, WK_DATA2D waves,
+    sponges, dispersion and breaking; FP32, production mode): one fw_step
+    against one oracle step, every array the production path keeps, bit for
+    bit (the oracle takes ~15 s for the step)."""
+    import fwcases
+    import fwgpu
+    import fwsetup
+    import oracle
+    print("\n[full-size parity] beach 2048 x 2048 sp, 1 step", flush=True)
+    st = fwsetup.build_state(fwcases.beach_2d(mglob=2048, nglob=2048, dx=2.0), "sp")
+    g = fwgpu.Solver(st, "sp")
+    gdt, gt = g.step(1)
+    g.synchronize()
+    o = oracle.OracleSolver(st)
+    o.run(1)
+    assert (gdt, gt) == (o.dt.value, o.time.value)
+    _compare(g, o, "after fw_step(1) at 2048^2", full_state=False)
+    g.close()
+
+
 PLAN = {"tiny2d": [1, 60, 250], "tiny1d": [1, 400], "island": [1, 120], "tiny2d_dx15": [1, 150],
         "config0": [1, 200], "member3": [1, 150], "tiny2d_fsponge": [1, 120], "tiny2d_nobrk": [1, 120],
         "tiny2d_swe": [1, 120]}
