@@ -173,18 +173,19 @@ def test_fluxes_with_slopes_beyond_quotient_range(prec, gpu_available):
     g.close()
 
 
-def test_full_size_beach_step_matches_oracle(gpu_available):
+@pytest.mark.parametrize("prec", ["sp", "dp"])
+def test_full_size_beach_step_matches_oracle(prec, gpu_available):
     """The benchmark workload itself (2048 x 2048 beach, WK_DATA2D waves,
-    sponges, dispersion and breaking; FP32, production mode): one fw_step
-    against one oracle step, every array the production path keeps, bit for
-    bit (the oracle takes ~15 s for the step)."""
+    sponges, dispersion and breaking; production mode): one fw_step against
+    one oracle step, every array the production path keeps, bit for bit
+    (the oracle takes 5-15 s for the step)."""
     import fwcases
     import fwgpu
     import fwsetup
     import oracle
-    print("\n[full-size parity] beach 2048 x 2048 sp, 1 step", flush=True)
-    st = fwsetup.build_state(fwcases.beach_2d(mglob=2048, nglob=2048, dx=2.0), "sp")
-    g = fwgpu.Solver(st, "sp")
+    print(f"\n[full-size parity] beach 2048 x 2048 {prec}, 1 step", flush=True)
+    st = fwsetup.build_state(fwcases.beach_2d(mglob=2048, nglob=2048, dx=2.0), prec)
+    g = fwgpu.Solver(st, prec)
     gdt, gt = g.step(1)
     g.synchronize()
     o = oracle.OracleSolver(st)
