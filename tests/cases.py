@@ -25,6 +25,21 @@ def tiny2d():
     return c
 
 
+def ragged2d():
+    """The 2-D beach at 75 x 53 cells: no grid dimension is a multiple of the
+    64-wide tiles, of the 16-line blocks or 64-row chunks of the line solver,
+    or of its 8-row groups (chunks of 11 and 53 rows leave 3 and 5 rows past
+    the last whole group)."""
+    c = fwcases.beach_2d(mglob=75, nglob=53, dx=2.0, nfreq=4, ndir=3, flat_depth=2.0,
+                         slope=1 / 25.0, x_toe=60.0, xc_wk=20.0, sponge=0.0)
+    c.params["Sponge_west_width"] = 10.0
+    c.params["Sponge_south_width"] = 8.0
+    c.params["Sponge_north_width"] = 8.0
+    c.spectrum["amp"] = c.spectrum["amp"] * 4.0
+    c.name = "ragged2d"
+    return c
+
+
 def tiny1d():
     """Quasi-1D periodic beach with the WK_REG source (reduced config 0)."""
     c = fwcases.beach_1d(mglob=200, nglob=4, xslp=80, xc_wk=40, sponge_west=20,
