@@ -40,6 +40,22 @@ def ragged2d():
     return c
 
 
+def periodic2d():
+    """The periodic WK_REG beach at 120 x 40 cells (dx = 1 m) with a 1:5 slope
+    that dries its east end and a Gaussian shoal off the centre line, so that
+    V varies along y and the periodic y line solves (Sherman-Morrison, 40
+    rows) carry real data."""
+    import numpy as np
+    m, n = 120, 40
+    c = fwcases.beach_1d(mglob=m, nglob=n, xslp=60, xc_wk=30, sponge_west=15)
+    X, Y = np.meshgrid(np.arange(m) + 0.5, np.arange(n) + 0.5)
+    depth = np.where(X > 60, 10.0 - (X - 60) / 5.0, 10.0)
+    c.depth = depth - 3.0 * np.exp(-((X - 75) ** 2 + (Y - 20) ** 2) / (2 * 6.0 ** 2))
+    c.params.update({"DEPTH_TYPE": "DATA", "DEPTH_FILE": "depth.txt"})
+    c.name = "periodic2d"
+    return c
+
+
 def tiny1d():
     """Quasi-1D periodic beach with the WK_REG source (reduced config 0)."""
     c = fwcases.beach_1d(mglob=200, nglob=4, xslp=80, xc_wk=40, sponge_west=20,

@@ -96,6 +96,32 @@ def _routine_steps(g, o, init, cmp, nsteps):
 
 
 @pytest.mark.parametrize("prec", ["sp", "dp"])
+def test_periodic_2d_matches_oracle(prec, gpu_available):
+    """A periodic domain 40 rows tall with y variation (shoal off the centre
+    line), built from the case description: 100 fused steps (every array bit
+    for bit after steps 1 and 100), then two steps routine by routine."""
+    import cases
+    import fwgpu
+    import fwsetup
+    import oracle
+    print(f"\n[periodic 2-D parity] 120 x 40 {prec}", flush=True)
+    st = fwsetup.build_state(cases.periodic2d(), prec)
+    assert st["PERIODIC"]
+    g = fwgpu.Solver(st, prec, full_state=True)
+    o = oracle.OracleSolver(st)
+    for n in (1, 99):
+        gdt, gt = g.step(n)
+        o.run(n)
+        assert (gdt, gt) == (o.dt.value, o.time.value)
+        _compare(g, o, f"after {n} more fused steps")
+    v = o["V"]
+    assert np.abs(v).max() > 1e-3 and np.abs(np.diff(v, axis=0)).max() > 1e-4  # V varies along y
+    assert int((o["MASK"] == 0).sum()) > 0
+    _routine_steps(g, o, st, lambda where: _compare(g, o, where), 2)
+    g.close()
+
+
+@pytest.mark.parametrize("prec", ["sp", "dp"])
 def test_ragged_grid_matches_oracle(prec, gpu_available):
     """75 x 53 cells, built from the case description (no fixture): partial
     tiles, partial blocks of lines, partial 64-row chunks and rows past the
