@@ -100,6 +100,29 @@ def member3():
     return c
 
 
+def tiny2d_dy1():
+    """The 2-D beach with cells of 2 m x 1 m: both spacings powers of two
+    (reciprocal multiplications) but different, so an x spacing used for y
+    or the reverse changes the result.  The waves keep the spectrum's own
+    amplitude: at four times it, troughs in the west sponge nearly dry out
+    and reach the Froude cap, whose per-cell ATAN2/COS/SIN differ between
+    device and host in the last bit."""
+    c = tiny2d()
+    c.params["DY"] = 1.0
+    c.spectrum["amp"] = c.spectrum["amp"] / 4.0
+    c.name = "tiny2d_dy1"
+    return c
+
+
+def tiny2d_dy15():
+    """The 2-D beach with cells of 2 m x 1.5 m: the division path with
+    different x and y spacings (spectrum amplitude as in tiny2d_dy1)."""
+    c = tiny2d_dy1()
+    c.params["DY"] = 1.5
+    c.name = "tiny2d_dy15"
+    return c
+
+
 def tiny2d_fsponge():
     """The 2-D beach with friction and diffusion sponges in place of the
     direct one and with bottom friction (sources.F friction, friction-sponge
@@ -133,4 +156,5 @@ def tiny2d_swe():
 
 CASES = {"tiny2d": tiny2d, "tiny1d": tiny1d, "island": island, "tiny2d_dx15": tiny2d_dx15,
          "config0": config0, "member3": member3, "tiny2d_fsponge": tiny2d_fsponge,
-         "tiny2d_nobrk": tiny2d_nobrk, "tiny2d_swe": tiny2d_swe}
+         "tiny2d_nobrk": tiny2d_nobrk, "tiny2d_swe": tiny2d_swe, "tiny2d_dy1": tiny2d_dy1,
+         "tiny2d_dy15": tiny2d_dy15}

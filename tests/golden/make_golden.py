@@ -28,7 +28,9 @@ PLAN = {"tiny2d": (["dp", "sp"], [1, 60, 250]),
         "member3": (["dp", "sp"], [1, 150]),
         "tiny2d_fsponge": (["dp", "sp"], [1, 120]),
         "tiny2d_nobrk": (["dp", "sp"], [1, 120]),
-        "tiny2d_swe": (["dp", "sp"], [1, 120])}
+        "tiny2d_swe": (["dp", "sp"], [1, 120]),
+        "tiny2d_dy1": (["dp", "sp"], [1, 120]),
+        "tiny2d_dy15": (["dp", "sp"], [1, 120])}
 
 
 def pack(d, keys=None):

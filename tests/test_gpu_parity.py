@@ -14,7 +14,7 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not oracle_available(), reason="oracle not built")]
 
 CASES = ["tiny2d", "tiny1d", "island", "tiny2d_dx15", "config0", "member3", "tiny2d_fsponge", "tiny2d_nobrk",
-         "tiny2d_swe"]
+         "tiny2d_swe", "tiny2d_dy1", "tiny2d_dy15"]
 
 
 def _solvers(name, prec, full_state=True):
@@ -254,7 +254,7 @@ def test_full_size_beach_step_matches_oracle(prec, gpu_available):
 
 PLAN = {"tiny2d": [1, 60, 250], "tiny1d": [1, 400], "island": [1, 120], "tiny2d_dx15": [1, 150],
         "config0": [1, 200], "member3": [1, 150], "tiny2d_fsponge": [1, 120], "tiny2d_nobrk": [1, 120],
-        "tiny2d_swe": [1, 120]}
+        "tiny2d_swe": [1, 120], "tiny2d_dy1": [1, 120], "tiny2d_dy15": [1, 120]}
 
 
 TOL = {"sp": 1e-5, "dp": 1e-10}

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.skipif(not oracle_available(), reason="oracle not built
 
 PLAN = {"tiny2d": [1, 60, 250], "tiny1d": [1, 400], "island": [1, 120], "tiny2d_dx15": [1, 150],
         "config0": [1, 200], "member3": [1, 150], "tiny2d_fsponge": [1, 120], "tiny2d_nobrk": [1, 120],
-        "tiny2d_swe": [1, 120]}
+        "tiny2d_swe": [1, 120], "tiny2d_dy1": [1, 120], "tiny2d_dy15": [1, 120]}
 
 
 @pytest.mark.parametrize("prec", ["dp", "sp"])
