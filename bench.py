@@ -53,9 +53,11 @@ KERNEL_TRAFFIC = {
     "brk1": (8, 0, 2, "cell"),
     "mask_mul": (6, 1, 5, "cell"),
 }
-# kernels whose time is set by HBM traffic, and the limiter of the others
-HBM_BOUND = ("src_huv", "huv", "sources", "disp_terms", "disp_deriv", "uv_final", "mask", "brk1", "mask_mul")
-LIMITER = {"flux_x": "valu", "flux_y": "valu", "trid": "latency (serial recurrences)"}
+# limiter of the kernels whose time is not set by HBM traffic (measured:
+# VALU busy and achieved bytes in the committed counter profiles)
+LIMITER = {"flux_x": "valu", "flux_y": "valu", "trid": "latency (serial recurrences)",
+           "brk2": "latency (one workgroup)", "dt_final": "latency (one workgroup)",
+           "max_final": "latency (one workgroup)"}
 
 
 def parse():
@@ -77,6 +79,11 @@ def parse():
     ap.add_argument("--fine-steps", type=int, default=200)
     ap.add_argument("--workers", type=int, default=8,
                     help="ensemble members run at a time (host threads)")
+    ap.add_argument("--batch", action="store_true",
+                    help="batched ensemble instead: members of uneven size in one launch per phase")
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--cpu-members-steps", type=int, default=3)
     return ap.parse_args()
 
 
@@ -126,6 +133,136 @@ def ensemble(args):
                    "data_s": round(t_data, 1), "train_s": round(t_train, 1)},
         "members": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in m.items()} for m in r["members"]],
     }
+    print(json.dumps(out))
+
+
+def batch_states(n_members, precision):
+    """The batched-ensemble workload: member m (seed 1000+m) with Mglob ~
+    U[256,1024], Nglob ~ U[128,768], dx = 2 m, a planar slope ~ U[1:20,1:60]
+    plus 1-4 Gaussian bumps, WK_REG waves of amplitude ~ U[0.2,1.0] m and
+    period ~ U[6,14] s, a fixed time step of half the initial CFL step
+    (rounded down to 1 ms)."""
+    import numpy as np
+    import fwcases
+    import fwgpu
+    import fwsetup
+    cases, states = [], []
+    for m in range(n_members):
+        c = fwcases.member(m)
+        st = fwsetup.build_state(c, precision)
+        g = fwgpu.Solver(st, precision)
+        dt, _ = g.estimate_dt()
+        g.close()
+        c.params["DT_fixed"] = float(np.floor(500.0 * dt) / 1000.0)
+        cases.append(c)
+        states.append(fwsetup.build_state(c, precision))
+    return cases, states
+
+
+def cpu_ensemble_baseline(cases, precision, steps, cores):
+    """The compiled reference on the host cores: `cores` members at once,
+    one serial process per core (each its own member of the same ensemble),
+    `steps` timed steps each after one warm-up step; aggregate cell-steps
+    per second over the concurrent members."""
+    ref = os.path.join(REPO, "oracle", "_ref", f"fwref_{precision}")
+    if not os.path.exists(ref):
+        return None
+    import shutil
+    procs, dirs = [], []
+    t0 = time.time()
+    for c in cases[:cores]:
+        d = tempfile.mkdtemp()
+        c.write(d)
+        dirs.append(d)
+        env = dict(os.environ, FWREF_NSTEPS=str(1 + steps), FWREF_WARMUP="1", MALLOC_PERTURB_="255",
+                   OMP_NUM_THREADS="1")
+        procs.append(subprocess.Popen(["bash", "-c", f"ulimit -s unlimited; exec {ref} input.txt"], cwd=d, env=env,
+                                      stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True))
+    import re
+    rate = 0.0
+    per_member = []
+    for c, p in zip(cases, procs):
+        out, _ = p.communicate(timeout=900)
+        line = [l for l in out.splitlines() if l.startswith("FWREF_TIMING")]
+        if p.returncode != 0 or not line:
+            return {"error": f"reference run failed rc={p.returncode}"}
+        mt = re.search(r"steps=\s*(\d+)\s+seconds=\s*([0-9.Ee+-]+)", line[0])
+        n, secs = int(mt.group(1)), float(mt.group(2))
+        cells = c.params["Mglob"] * c.params["Nglob"]
+        rate += cells * n / secs
+        per_member.append(secs / n)
+    for d in dirs:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"value": round(rate / 1e6, 4), "unit": "Mcell-steps/s", "cores": len(procs), "kind": "reference",
+            "sample": f"reference FUNWAVE-TVD ({'FP64' if precision == 'dp' else 'FP32'} serial build) on the first "
+                      f"{len(procs)} members of the same ensemble, one process per core, all at once: {steps} steps "
+                      f"each after 1 warm-up step ({time.time() - t0:.1f} s with set-up); mean "
+                      f"{sum(per_member) / len(per_member):.3f} s per member-step"}
+
+
+def batch_bench(args):
+    """Batched ensemble: --members members of uneven size (batch_states)
+    advanced --batch-steps whole steps through fw_batch (one launch per
+    phase over all members, one graph per step, autotuned shapes), against
+    the same members one after another with fw_step, and the reference on
+    the host cores."""
+    import fwgpu
+    rb = 8 if args.precision == "dp" else 4
+    t_setup = time.time()
+    cases, states = batch_states(args.members, args.precision)
+    gs = [fwgpu.Solver(st, args.precision) for st in states]
+    cells = [g.cfg.Mglob * g.cfg.Nglob for g in gs]
+    b = fwgpu.Batch(gs)
+    report = b.autotune(trial_steps=2)
+    t_setup = time.time() - t_setup
+    b.step(args.warmup)
+    sync()
+    t0 = time.perf_counter()
+    dev_ms = b.time_steps(args.batch_steps)
+    sync()
+    wall = time.perf_counter() - t0
+    assert b.status() == [0] * len(gs), b.status()
+    tot = sum(cells)
+    mcs = tot * args.batch_steps / wall / 1e6
+    # the same members one at a time (fw_step, one context each), a few steps
+    k = 10
+    t1 = time.perf_counter()
+    for g in gs:
+        g.step(k)
+        g.synchronize()
+    seq = time.perf_counter() - t1
+    seq_mcs = tot * k / seq / 1e6
+    step_s = wall / args.batch_steps
+    alg = B_ALG_WORDS * rb * tot
+    out = {
+        "metric": "Mcell-steps/s, batched ensemble (all members)", "value": round(mcs, 3), "unit": "Mcell-steps/s",
+        "n_gpus": 1, "steps": args.batch_steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64" if rb == 8 else "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(gs)} ensemble members (seeds 1000..{999 + len(gs)}), Mglob~U[256,1024], "
+                               "Nglob~U[128,768], dx=2 m, slope + bumps, WK_REG waves, west sponge, dispersion + "
+                               "breaking, FIXED_DT; one launch per phase over all members, one hipGraph per step",
+                   "members": len(gs), "cells_total": tot, "cells_min": min(cells), "cells_max": max(cells),
+                   "parallelism": "all members on one GPU"},
+        "roofline": {"bound": "hbm", "kernel": "whole step of every member",
+                     "achieved": round(alg / step_s / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(alg / step_s / HBM_PEAK, 4), "traffic": None,
+                     "alg_bytes_per_cell_step": B_ALG_WORDS * rb,
+                     "device_ms_per_step": round(dev_ms / args.batch_steps, 4)},
+        "ensemble_runs_per_hour": round(3600.0 / (step_s * args.batch_steps) * len(gs), 1),
+        "members_one_at_a_time_Mcell_steps_s": round(seq_mcs, 3),
+        "speedup_vs_one_at_a_time": round(mcs / seq_mcs, 3),
+        "autotune": report.strip().splitlines()[-4:], "setup_s": round(t_setup, 1),
+    }
+    if not args.no_cpu_baseline:
+        cores = min(16, os.cpu_count() or 1)
+        cb = cpu_ensemble_baseline(cases, args.precision, args.cpu_members_steps, cores)
+        if cb and "value" in cb:  # member runs (of the mean member size) per hour
+            cb["ensemble_runs_per_hour"] = round(3600.0 * cb["value"] * 1e6 / (tot / len(gs) * args.batch_steps), 3)
+        out["cpu_baseline"] = cb
+    b.close()
+    for g in gs:
+        g.close()
     print(json.dumps(out))
 
 
@@ -234,6 +371,9 @@ def main():
     if args.ensemble:
         ensemble(args)
         return
+    if args.batch:
+        batch_bench(args)
+        return
     import numpy as np  # noqa: F401
     import fwcases
     import fwgpu
@@ -268,19 +408,28 @@ def main():
         cnt, ms = g.profile_get(name)
         kern[name] = (cnt, ms)
     g.profile(False)
-    tot = sum(v[1] for v in kern.values())
-    # The roofline object covers the dominant HBM-bound kernel; the flux
-    # kernels are VALU-bound and the line solver latency-bound (their
-    # limiters are in the committed counter profile).
-    dom = max((k for k in kern if k in HBM_BOUND), key=lambda k: kern[k][1])
-    top = max(kern, key=lambda k: kern[k][1])
-    cnt, ms = kern[dom]
-    avg_s = ms / cnt / 1e3
-    alg = kernel_traffic_bytes(dom, rb, M, N, n, n)
-    achieved = alg / avg_s
-    traffic, traffic_src = measured_traffic(dom, rb, n)
     step_s = wall / args.steps
     step_alg = B_ALG_WORDS * rb * n * n
+    # Every kernel of the step, longest first: device ms per step, share of
+    # the summed kernel time, its algorithmic bytes per launch over its
+    # average launch time, and its limiter (from the committed counters).
+    ksum = sum(v[1] for v in kern.values())
+    breakdown = []
+    traffic_step, traffic_src = 0.0, None
+    for name, (cnt, ms) in sorted(kern.items(), key=lambda kv: -kv[1][1]):
+        row = {"kernel": name, "ms_per_step": round(ms / args.profile_steps, 4),
+               "launches_per_step": round(cnt / args.profile_steps, 2), "share": round(ms / ksum, 4),
+               "bound": LIMITER.get(name, "hbm")}
+        if name in KERNEL_TRAFFIC:
+            alg = kernel_traffic_bytes(name, rb, M, N, n, n)
+            row["alg_GBps"] = round(alg / (ms / cnt / 1e3) / 1e9, 1)
+            row["alg_frac_of_hbm_peak"] = round(alg / (ms / cnt / 1e3) / HBM_PEAK, 4)
+        t, src = measured_traffic(name, rb, n)
+        if t is not None:
+            row["counter_MB_per_launch"] = round(t / 1e6, 1)
+            traffic_step += t * cnt / args.profile_steps
+            traffic_src = src
+        breakdown.append(row)
     out = {
         "metric": METRIC, "value": round(mcs, 3), "unit": "Mcell-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -290,18 +439,22 @@ def main():
                                f"waves, {n}x{n} cells dx=2 m, sponges W/S/N, wetting-drying, "
                                f"dispersion + breaking, whole RK3 steps",
                    "Mglob": n, "Nglob": n, "precision": "FP64" if rb == 8 else "FP32",
+                   "line_solver": "thomas (bitwise the reference's recurrence)",
                    "parallelism": "one domain on one GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2),
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": traffic, "traffic_source": traffic_src,
-                     "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": alg,
-                     "longest_kernel": top, "longest_kernel_bound": LIMITER.get(top, "hbm")},
-        "step_roofline": {"alg_bytes_per_cell_step": B_ALG_WORDS * rb,
-                          "achieved_GBps": round(step_alg / step_s / 1e9, 2),
-                          "frac": round(step_alg / step_s / HBM_PEAK, 4),
-                          "device_ms_per_step": round(dev_ms / args.steps, 4)},
-        "kernels_ms_per_step": {k: round(v[1] / args.profile_steps, 4) for k, v in
-                                sorted(kern.items(), key=lambda kv: -kv[1][1])},
+        # The whole step against its canonical algorithmic traffic:
+        # 93 words per interior cell-step (372 B FP32, 744 B FP64), i.e.
+        # the fields that must cross HBM when every stage-local intermediate
+        # stays on chip; traffic = the rocprofv3 counter bytes of one step
+        # (every kernel's calibrated FETCH_SIZE + WRITE_SIZE per launch x its
+        # launches per step).
+        "roofline": {"bound": "hbm", "kernel": "whole step (every kernel of ESTIMATE_DT + 3 RK stages)",
+                     "achieved": round(step_alg / step_s / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(step_alg / step_s / HBM_PEAK, 4),
+                     "traffic": round(traffic_step) if traffic_src else None,
+                     "traffic_source": traffic_src,
+                     "alg_bytes_per_step": step_alg, "alg_bytes_per_cell_step": B_ALG_WORDS * rb,
+                     "device_ms_per_step": round(dev_ms / args.steps, 4)},
+        "kernels": breakdown,
         "setup_s": round(t_setup, 1),
     }
     out["learned_correction"] = cnn_roofline()

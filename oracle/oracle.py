@@ -183,12 +183,17 @@ def ref_binary(precision: str) -> str:
 
 def run_reference(case_dir: str, precision: str, nsteps: int, prefix="d",
                   dump_init=True, dump_every=0, trace_step=-1, warmup=-1,
-                  output=False, timeout=600):
-    """Run the compiled reference in ``case_dir`` (containing input.txt)."""
+                  output=False, timeout=600, dump_list=(), until_total=False):
+    """Run the compiled reference in ``case_dir`` (containing input.txt).
+    ``dump_list``: steps after which the state is dumped; ``until_total``:
+    also stop once TIME >= TOTAL_TIME (src/main.F:284) and dump the final
+    state as <prefix>_final.bin."""
     env = dict(os.environ)
     env.update({"FWREF_PREFIX": prefix, "FWREF_NSTEPS": str(nsteps),
                 "FWREF_DUMP_INIT": "1" if dump_init else "0",
                 "FWREF_DUMP_EVERY": str(dump_every),
+                "FWREF_DUMP_LIST": ",".join(str(int(s)) for s in dump_list),
+                "FWREF_UNTIL_TOTAL": "1" if until_total else "0",
                 "FWREF_TRACE_STEP": str(trace_step),
                 "FWREF_WARMUP": str(warmup),
                 "FWREF_OUTPUT": "1" if output else "0",

@@ -14,6 +14,12 @@
 !   FWREF_NSTEPS      number of time steps to run (default 0)
 !   FWREF_DUMP_INIT   1 -> write <prefix>_init.bin after INITIALIZATION
 !   FWREF_DUMP_EVERY  k -> write <prefix>_step<NNNNNN>.bin every k steps
+!   FWREF_DUMP_LIST   comma-separated steps (at most 32) -> write
+!                     <prefix>_step<NNNNNN>.bin after each of them
+!   FWREF_UNTIL_TOTAL 1 -> also stop once TIME >= TOTAL_TIME (the loop
+!                     condition of src/main.F:284) and write
+!                     <prefix>_final.bin after the last step, and print
+!                     FWREF_STEPS=<steps run>
 !   FWREF_TRACE_STEP  s -> in step s write <prefix>_trace_<stage>_<routine>.bin
 !                     after every routine of every stage
 !   FWREF_OUTPUT      1 -> call the reference OUTPUT at the top of each step
@@ -27,7 +33,9 @@ PROGRAM FWREF_DRIVER
   USE FWREF_DUMP
   IMPLICIT NONE
   INTEGER :: nsteps, dump_init, dump_every, trace_step, do_output, warmup
-  INTEGER :: istep, istg
+  INTEGER :: istep, istg, until_total, ndl, kd, ndone
+  INTEGER :: dump_list(32)
+  LOGICAL :: dump_now
   INTEGER(8) :: c0, c1, crate
   CHARACTER(LEN=256) :: prefix, fname
   LOGICAL :: tr
@@ -39,6 +47,8 @@ PROGRAM FWREF_DRIVER
   trace_step = env_int('FWREF_TRACE_STEP', -1)
   do_output  = env_int('FWREF_OUTPUT', 0)
   warmup     = env_int('FWREF_WARMUP', -1)
+  until_total = env_int('FWREF_UNTIL_TOTAL', 0)
+  CALL env_list('FWREF_DUMP_LIST', dump_list, ndl)
 
   ! initialisation, src/main.F:177-214
   CALL READ_INPUT
@@ -49,7 +59,10 @@ PROGRAM FWREF_DRIVER
 
   IF (dump_init == 1) CALL dump_state(TRIM(prefix)//'_init.bin')
 
+  ndone = 0
   DO istep = 1, nsteps
+    IF (until_total == 1 .AND. TIME >= TOTAL_TIME) EXIT
+    ndone = istep
     IF (istep == warmup + 1) CALL SYSTEM_CLOCK(c0, crate)
     tr = (istep == trace_step)
     IF (do_output == 1) CALL OUTPUT
@@ -84,13 +97,20 @@ PROGRAM FWREF_DRIVER
         IF (tr) CALL trace(istg, 'spg')
       ENDIF
     ENDDO
-    IF (dump_every > 0) THEN
-      IF (MOD(istep, dump_every) == 0) THEN
-        WRITE(fname, '(A,A,I6.6,A)') TRIM(prefix), '_step', istep, '.bin'
-        CALL dump_state(TRIM(fname))
-      ENDIF
+    dump_now = .FALSE.
+    IF (dump_every > 0) dump_now = MOD(istep, dump_every) == 0
+    DO kd = 1, ndl
+      IF (dump_list(kd) == istep) dump_now = .TRUE.
+    ENDDO
+    IF (dump_now) THEN
+      WRITE(fname, '(A,A,I6.6,A)') TRIM(prefix), '_step', istep, '.bin'
+      CALL dump_state(TRIM(fname))
     ENDIF
   ENDDO
+  IF (until_total == 1) THEN
+    CALL dump_state(TRIM(prefix)//'_final.bin')
+    WRITE(*,'(A,I0)') 'FWREF_STEPS=', ndone
+  ENDIF
 
   IF (warmup >= 0 .AND. nsteps > warmup) THEN
     CALL SYSTEM_CLOCK(c1)
@@ -120,6 +140,30 @@ CONTAINS
       READ(v, *) env_int
     ENDIF
   END FUNCTION env_int
+
+  SUBROUTINE env_list(name, vals, n)
+    CHARACTER(LEN=*), INTENT(IN) :: name
+    INTEGER, INTENT(OUT) :: vals(:), n
+    CHARACTER(LEN=512) :: v
+    INTEGER :: ln, st, q, a
+    n = 0
+    CALL GET_ENVIRONMENT_VARIABLE(name, v, ln, st)
+    IF (st /= 0 .OR. ln == 0) RETURN
+    a = 1
+    DO q = 1, ln + 1
+      IF (q > ln) THEN
+        IF (q > a) THEN
+          n = n + 1
+          READ(v(a:q-1), *) vals(n)
+        ENDIF
+      ELSE IF (v(q:q) == ',') THEN
+        n = n + 1
+        READ(v(a:q-1), *) vals(n)
+        a = q + 1
+      ENDIF
+      IF (n == SIZE(vals)) EXIT
+    ENDDO
+  END SUBROUTINE env_list
 
   SUBROUTINE env_str(name, dflt, out)
     CHARACTER(LEN=*), INTENT(IN) :: name, dflt
