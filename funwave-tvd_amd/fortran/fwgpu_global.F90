@@ -58,10 +58,29 @@ CONTAINS
     CALL fwgpu_check(fw_download(fwctx, fwgpu_id(name), C_LOC(a)), 'download '//name)
   END SUBROUTINE get_i
 
+  ! Inputs the reference reads at run time on this path that have no device
+  ! counterpart: each stops the program here instead of letting the device
+  ! run different physics.  (Time_Scheme is read by READ_INPUT but no
+  ! routine of the step reads it, src/io.F:2806; the step is always SSP-RK3.)
+  SUBROUTINE FWGPU_REFUSE_UNSUPPORTED
+    IF (WAVEMAKER_VIS) CALL refuse('WAVEMAKER_VIS = T (breaker.F VISCOSITY_WMAKER, sources.F nu terms)')
+    IF (BREAKWATER) CALL refuse('BREAKWATER = T (sponge-like breakwater friction, init.F/sources.F)')
+    IF (WaveMakerCurrentBalance) CALL refuse('WaveMakerCurrentBalance = T (sources.F wavemaker current balance)')
+    IF (TRIM(HIGH_ORDER) /= 'FOURTH') CALL refuse('HIGH_ORDER = '//TRIM(HIGH_ORDER)//' (only FOURTH, fluxes.F:1324)')
+    IF (CONSTR(1:3) /= 'HLL') CALL refuse('CONSTRUCTION = '//TRIM(CONSTR)//' (only HLL, fluxes.F:109)')
+  END SUBROUTINE FWGPU_REFUSE_UNSUPPORTED
+
+  SUBROUTINE refuse(what)
+    CHARACTER(LEN=*), INTENT(IN) :: what
+    WRITE(*,*) 'FWGPU: unsupported on the device path: ', what
+    STOP 2
+  END SUBROUTINE refuse
+
   ! Create the device domain from MODULE GLOBAL (after INITIALIZATION,
   ! src/main.F:214) and upload the whole per-step state.
   SUBROUTINE FWGPU_INIT
     TYPE(fw_config) :: c
+    CALL FWGPU_REFUSE_UNSUPPORTED
     c%real_bytes = STORAGE_SIZE(DX) / 8
     c%Mloc = Mloc; c%Nloc = Nloc; c%Mglob = Mglob; c%Nglob = Nglob
     c%Ibeg = Ibeg; c%Iend = Iend; c%Jbeg = Jbeg; c%Jend = Jend
@@ -79,8 +98,7 @@ CONTAINS
     ELSEIF (WaveMaker(1:4) == 'INI_' .OR. WaveMaker(1:6) == 'N_WAVE') THEN
       c%wavemaker = FW_WK_NONE
     ELSE
-      WRITE(*,*) 'FWGPU: wavemaker ', TRIM(WaveMaker), ' has no device path'
-      STOP
+      CALL refuse('WAVEMAKER = '//TRIM(WaveMaker))
     ENDIF
     c%NumFreq = NumFreq; c%ilo = ilo; c%ihi = ihi; c%jlo = jlo; c%jhi = jhi
     c%DX = DX; c%DY = DY; c%DXg = DXg; c%DYg = DYg; c%CFL = CFL
