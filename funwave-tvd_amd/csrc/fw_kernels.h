@@ -29,10 +29,22 @@ namespace fw {
 struct Tile {
   int x, y;
 };
-__device__ __forceinline__ Tile xcd_tile() {
-  const int gx = gridDim.x;
-  const int nb = gx * gridDim.y;
-  const int h = blockIdx.x + blockIdx.y * gx;
+
+// The workgroup's place in its launch: tile (x, y) of a gx x gy grid.  A
+// plain launch takes it from blockIdx/gridDim (grid_blk); a launch over the
+// members of an ensemble takes it from its tile queue (the member's own
+// grid), already in the XCD order below (remap = 0).
+struct Blk {
+  int x, y, gx, gy, remap;
+};
+__device__ __forceinline__ Blk grid_blk() {
+  return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y, 1};
+}
+__device__ __forceinline__ Tile xcd_tile(const Blk &B) {
+  if (!B.remap) return Tile{B.x, B.y};
+  const int gx = B.gx;
+  const int nb = gx * B.gy;
+  const int h = B.x + B.y * gx;
   const int xcd = h & 7, k = h >> 3;
   const int per = nb >> 3, rem = nb & 7;
   const int lin = xcd * per + min(xcd, rem) + k;
@@ -149,40 +161,41 @@ __device__ __forceinline__ bool halo_pos(const HaloSpec &s, int t, int mode, int
   return false;
 }
 
+// Ghost fill of one array A; the workgroup's x tile picks its ghost cells.
 template <class R>
+__device__ __forceinline__ void halo_body(const HaloArr &A, Geo g, int mode, const Blk &B) {
+  int t = B.x * blockDim.x + threadIdx.x;
+  HaloSpec s = halo_spec(g, A.spec);
+  int i, j;
+  if (!halo_pos(s, t, mode, i, j)) return;
+  int si = i, sj = j;
+  bool neg = false;
+  if (mode == 1) {  // periodic y copy (raw, no sign)
+    sj = (j <= 3) ? (s.Je - 3 + j) : (3 + (j - s.Je));
+  } else {
+    if (i < s.Ib) { si = 2 * s.Ib - 1 - i; neg ^= (A.vtype == 2); }
+    if (i > s.Ie) { si = 2 * s.Ie + 1 - i; neg ^= (A.vtype == 2); }
+    if (mode == 0) {
+      if (j < s.Jb) { sj = 2 * s.Jb - 1 - j; neg ^= (A.vtype == 3); }
+      if (j > s.Je) { sj = 2 * s.Je + 1 - j; neg ^= (A.vtype == 3); }
+    }
+  }
+  if (A.isint) {
+    int *ph = (int *)A.p;
+    ph[IX(i, j)] = ph[IX(si, sj)];
+  } else {
+    R *ph = (R *)A.p;
+    R v = ph[IX(si, sj)];
+    ph[IX(i, j)] = neg ? -v : v;
+  }
+}
 // One array of the list per grid row (blockIdx.y): within a mode every ghost
 // value is read from interior cells only, so the arrays are independent and
 // their fills run side by side instead of one load-store round trip after
 // another.
+template <class R>
 __global__ void k_halo(HaloList L, Geo g, int mode) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  {
-    const int q = blockIdx.y;
-    const HaloArr A = L.a[q];
-    HaloSpec s = halo_spec(g, A.spec);
-    int i, j;
-    if (!halo_pos(s, t, mode, i, j)) return;
-    int si = i, sj = j;
-    bool neg = false;
-    if (mode == 1) {  // periodic y copy (raw, no sign)
-      sj = (j <= 3) ? (s.Je - 3 + j) : (3 + (j - s.Je));
-    } else {
-      if (i < s.Ib) { si = 2 * s.Ib - 1 - i; neg ^= (A.vtype == 2); }
-      if (i > s.Ie) { si = 2 * s.Ie + 1 - i; neg ^= (A.vtype == 2); }
-      if (mode == 0) {
-        if (j < s.Jb) { sj = 2 * s.Jb - 1 - j; neg ^= (A.vtype == 3); }
-        if (j > s.Je) { sj = 2 * s.Je + 1 - j; neg ^= (A.vtype == 3); }
-      }
-    }
-    if (A.isint) {
-      int *ph = (int *)A.p;
-      ph[IX(i, j)] = ph[IX(si, sj)];
-    } else {
-      R *ph = (R *)A.p;
-      R v = ph[IX(si, sj)];
-      ph[IX(i, j)] = neg ? -v : v;
-    }
-  }
+  halo_body<R>(L.a[blockIdx.y], g, mode, grid_blk());
 }
 
 // --------------------------------------------------------------------------
@@ -190,7 +203,7 @@ __global__ void k_halo(HaloList L, Geo g, int mode) {
 // minima are reduced by k_dt_final.  Also U0 := U, V0 := V (main.F:331-332)
 // when copy_uv is set.
 template <class R>
-__global__ void k_dt_partial(Dev<R> d, Geo g, Prm<R> p, int copy_uv) {
+__device__ __forceinline__ void dt_partial_body(Dev<R> d, Geo g, Prm<R> p, int copy_uv, const Blk &B) {
   __shared__ R sm[256];
   R t3 = Lit<R>::large();
   // workgroups stride over rows, threads over the row (a minimum does not
@@ -198,7 +211,7 @@ __global__ void k_dt_partial(Dev<R> d, Geo g, Prm<R> p, int copy_uv) {
   // four cells per pass: loads first (clamped indices, unconditional), then
   // the copies and the minimum (a store would hold back later loads)
   constexpr int KB = 4;
-  for (int j = 1 + blockIdx.x; j <= g.N; j += gridDim.x)
+  for (int j = 1 + B.x; j <= g.N; j += B.gx)
     for (int i0 = 1 + threadIdx.x; i0 <= g.M; i0 += KB * blockDim.x) {
       R u[KB], v[KB], h[KB];
 #pragma unroll
@@ -227,11 +240,15 @@ __global__ void k_dt_partial(Dev<R> d, Geo g, Prm<R> p, int copy_uv) {
     if (threadIdx.x < s) sm[threadIdx.x] = rmin(sm[threadIdx.x], sm[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) d.red[blockIdx.x] = sm[0];
+  if (threadIdx.x == 0) d.red[B.x] = sm[0];
+}
+template <class R>
+__global__ void k_dt_partial(Dev<R> d, Geo g, Prm<R> p, int copy_uv) {
+  dt_partial_body<R>(d, g, p, copy_uv, grid_blk());
 }
 
 template <class R>
-__global__ void k_dt_final(Dev<R> d, Prm<R> p, int nblk, R *out) {
+__device__ __forceinline__ void dt_final_body(Dev<R> d, Prm<R> p, int nblk, R *out, const Blk &B) {
   __shared__ R sm[256];
   R m = Lit<R>::large();
   for (int q = threadIdx.x; q < nblk; q += blockDim.x) m = rmin(m, d.red[q]);
@@ -253,14 +270,18 @@ __global__ void k_dt_final(Dev<R> d, Prm<R> p, int nblk, R *out) {
     out[0] = dt;
   }
 }
+template <class R>
+__global__ void k_dt_final(Dev<R> d, Prm<R> p, int nblk, R *out) {
+  dt_final_body<R>(d, p, nblk, out, grid_blk());
+}
 
 // max |eta| over the interior (CHECK_BLOWUP, misc.F:394-433)
 template <class R>
-__global__ void k_maxeta_partial(Dev<R> d, Geo g) {
+__device__ __forceinline__ void maxeta_partial_body(Dev<R> d, Geo g, const Blk &B) {
   __shared__ R sm[256];
   R m = (R)0;
   // 8 loads in flight per thread before the comparisons
-  for (int j = g.Jb + blockIdx.x; j <= g.Je; j += gridDim.x)
+  for (int j = g.Jb + B.x; j <= g.Je; j += B.gx)
     for (int i0 = g.Ib + threadIdx.x; i0 <= g.Ie; i0 += 8 * blockDim.x) {
       R a[8];
 #pragma unroll
@@ -279,10 +300,14 @@ __global__ void k_maxeta_partial(Dev<R> d, Geo g) {
     if (threadIdx.x < s) sm[threadIdx.x] = rmax(sm[threadIdx.x], sm[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) d.red[blockIdx.x] = sm[0];
+  if (threadIdx.x == 0) d.red[B.x] = sm[0];
 }
 template <class R>
-__global__ void k_max_final(Dev<R> d, int nblk, R *out) {
+__global__ void k_maxeta_partial(Dev<R> d, Geo g) {
+  maxeta_partial_body<R>(d, g, grid_blk());
+}
+template <class R>
+__device__ __forceinline__ void max_final_body(Dev<R> d, int nblk, R *out, const Blk &B) {
   __shared__ R sm[256];
   R m = (R)0;
   for (int q = threadIdx.x; q < nblk; q += blockDim.x) m = rmax(m, d.red[q]);
@@ -293,6 +318,10 @@ __global__ void k_max_final(Dev<R> d, int nblk, R *out) {
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = sm[0];
+}
+template <class R>
+__global__ void k_max_final(Dev<R> d, int nblk, R *out) {
+  max_final_body<R>(d, nblk, out, grid_blk());
 }
 
 // plain copies of whole pitched arrays (Eta0 := Eta etc.)
@@ -359,8 +388,8 @@ struct WideLB {
   static constexpr int wg256 = sizeof(R) == 8 ? 256 : 1024;
 };
 template <class R, bool P2>
-__global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
-  const Tile tb = xcd_tile();
+__device__ __forceinline__ void disp_deriv_body(Dev<R> d, Geo g, Prm<R> p, R DT, const Blk &B) {
+  const Tile tb = xcd_tile(B);
   int i = 1 + tb.x * blockDim.x + threadIdx.x;
   int j = 1 + tb.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
@@ -448,12 +477,16 @@ __global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_deriv(Dev<R> d, Geo g
     d.DVtxy[o] = wall ? (R)0 : gfxy<R>(DVT, i, j, dx, dy) * m9;
   }
 }
+template <class R, bool P2>
+__global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_deriv(Dev<R> d, Geo g, Prm<R> p, R DT) {
+  disp_deriv_body<R, P2>(d, g, p, DT, grid_blk());
+}
 
 // CAL_DISPERSION part 2: U4, V4, U1p, V1p on all cells (:293-330) and the
 // nonlinear terms U1pp..V3 on the interior (:398-503).
 template <class R, bool P2>
-__global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_terms(Dev<R> d, Geo g, Prm<R> p, R DT) {
-  const Tile tb = xcd_tile();
+__device__ __forceinline__ void disp_terms_body(Dev<R> d, Geo g, Prm<R> p, R DT, const Blk &B) {
+  const Tile tb = xcd_tile(B);
   int i = 1 + tb.x * blockDim.x + threadIdx.x;
   int j = 1 + tb.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
@@ -588,6 +621,10 @@ __global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_terms(Dev<R> d, Geo g
   d.sumV[o] = v1pp + v2 + v3;
   store4();
 #undef A_
+}
+template <class R, bool P2>
+__global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_terms(Dev<R> d, Geo g, Prm<R> p, R DT) {
+  disp_terms_body<R, P2>(d, g, p, DT, grid_blk());
 }
 
 // --------------------------------------------------------------------------
@@ -902,10 +939,10 @@ __device__ __forceinline__ void xface_finish(const Dev<R> &d, const Geo &g, cons
 
 // blockDim (64, 4): threadIdx.x is the lane, threadIdx.y picks the row.
 template <class R, bool DISP>
-__global__ void __launch_bounds__(256) k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
+__device__ __forceinline__ void flux_x_body(Dev<R> d, Geo g, Prm<R> p, const Blk &B) {
   const int lane = threadIdx.x;
-  const int c = blockIdx.x * kFluxXFaces + lane;  // cell / face index of this lane
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int c = B.x * kFluxXFaces + lane;  // cell / face index of this lane
+  const int j = 1 + B.y * blockDim.y + threadIdx.y;
   if (j > g.N) return;  // whole wavefront
   const bool own = lane >= 1 && lane <= kFluxXFaces && c <= g.M + 1;
   const bool row = j >= g.Jb && j <= g.Je;
@@ -971,6 +1008,10 @@ __global__ void __launch_bounds__(256) k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
   }
   if (!comp || !DISP) { u4L = d.U4xL[o]; u4R = d.U4xR[o]; v4L = d.V4xL[o]; v4R = d.V4xR[o]; }
   xface_finish(d, g, p, c, j, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
+}
+template <class R, bool DISP>
+__global__ void __launch_bounds__(256) k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
+  flux_x_body<R, DISP>(d, g, p, grid_blk());
 }
 
 // y faces (CONSTRUCT_HO_Y fluxes.F:1642-1754, bc.F:147-210 and :221-293).
@@ -1053,14 +1094,14 @@ struct FluxYK {
 };
 
 template <class R, bool DISP, int K>
-__global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
+__device__ __forceinline__ void flux_y_body(Dev<R> d, Geo g, Prm<R> p, const Blk &B) {
   constexpr int NV = DISP ? 7 : 5;
   constexpr int ROWS = FluxYTile<K>::rows, FACES = FluxYTile<K>::faces;
   // per field: D(j) of every tile row, then (after the pairs are formed) the
   // left state of face j+1
   __shared__ R sX[NV][ROWS][64];
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i = 1 + blockIdx.x * 64 + tx;
+  const int i = 1 + B.x * 64 + tx;
   const int N1 = g.N + 1;
   const ptrdiff_t s = (ptrdiff_t)g.P;
   const bool col = i >= g.Ib && i <= g.Ie;
@@ -1069,7 +1110,7 @@ __global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo 
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const int r = ty + kFluxYThreadRows * k;
-    const int j = blockIdx.y * FACES + r;
+    const int j = B.y * FACES + r;
     // D(j) needs cells j-2..j+1
     const bool dok = col && j >= g.Jb - 1 && j <= g.Je + 2;
     const size_t o = IX(i <= g.M ? i : g.M, j < 1 ? 1 : (j > N1 ? N1 : j));
@@ -1103,7 +1144,7 @@ __global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo 
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const int r = ty + kFluxYThreadRows * k;
-    const int j = blockIdx.y * FACES + r;
+    const int j = B.y * FACES + r;
     // the pair of cell j needs D(j), D(j+1)
     const bool pok = col && j >= g.Jb - 1 && j <= g.Je + 1 && r + 1 < ROWS;
 #pragma unroll
@@ -1118,7 +1159,7 @@ __global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo 
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const int r = ty + kFluxYThreadRows * k;
-    const int j = blockIdx.y * FACES + r;
+    const int j = B.y * FACES + r;
     const bool own = r >= 1 && r <= FACES && j <= N1 && i <= g.M;
     if (!own) continue;
     const bool comp = col && j >= g.Jb && j <= g.Je + 1;
@@ -1142,6 +1183,10 @@ __global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo 
     yface_finish(d, g, p, i, j, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
   }
 }
+template <class R, bool DISP, int K>
+__global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
+  flux_y_body<R, DISP, K>(d, g, p, grid_blk());
+}
 
 // y faces by marching: each thread walks one column through a strip of RS
 // faces, faces jb+1..jb+RS, loading one cell (every field) per row into a
@@ -1155,10 +1200,10 @@ __global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo 
 // (five waves per SIMD: 96 VGPRs with a few spilled dwords beat four waves
 // at 103)
 template <class R, bool DISP, int RS>
-__global__ void __launch_bounds__(256, 5) k_flux_y_march(Dev<R> d, Geo g, Prm<R> p) {
+__device__ __forceinline__ void flux_y_march_body(Dev<R> d, Geo g, Prm<R> p, const Blk &B) {
   constexpr int NV = DISP ? 7 : 5;
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int jb = (blockIdx.y * blockDim.y + threadIdx.y) * RS;
+  const int i = 1 + B.x * 64 + threadIdx.x;
+  const int jb = (B.y * blockDim.y + threadIdx.y) * RS;
   const int N1 = g.N + 1;
   if (i > g.M || jb + 1 > N1) return;
   const bool col = i >= g.Ib && i <= g.Ie;
@@ -1244,6 +1289,10 @@ __global__ void __launch_bounds__(256, 5) k_flux_y_march(Dev<R> d, Geo g, Prm<R>
 #pragma unroll
     for (int q = 0; q < NV; q++) Dp[q] = Dn[q];
   }
+}
+template <class R, bool DISP, int RS>
+__global__ void __launch_bounds__(256, 5) k_flux_y_march(Dev<R> d, Geo g, Prm<R> p) {
+  flux_y_march_body<R, DISP, RS>(d, g, p, grid_blk());
 }
 
 // --------------------------------------------------------------------------
@@ -1391,8 +1440,8 @@ __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
 // read back, which is the same arithmetic on the same inputs: SourceTerms
 // reads nothing ESTIMATE_HUV writes.
 template <class R, bool FUSED, bool P2>
-__global__ void __launch_bounds__(WideLB<R>::wg256) k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA) {
-  const Tile tb = xcd_tile();
+__device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA, const Blk &B) {
+  const Tile tb = xcd_tile(B);
   int i = 1 + tb.x * blockDim.x + threadIdx.x;
   int j = 1 + tb.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
@@ -1494,6 +1543,10 @@ __global__ void __launch_bounds__(WideLB<R>::wg256) k_huv(Dev<R> d, Geo g, Prm<R
     d.H[o] = eta * p.G3 + d.Depth[o];
   }
 }
+template <class R, bool FUSED, bool P2>
+__global__ void __launch_bounds__(WideLB<R>::wg256) k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA) {
+  huv_body<R, FUSED, P2>(d, g, p, al, be, DT, AA, grid_blk());
+}
 
 // TRIDx_ser / TRIDy_ser (tridiagnal.F:338-398): Thomas sweeps on rows
 // normalised by the diagonal; a row with A == 0 is left as it is.  One
@@ -1527,15 +1580,15 @@ __device__ __forceinline__ void thomas(R *A, R *C, R *D, R *Z, size_t o0, ptrdif
 }
 
 template <class R>
-__global__ void k_trid(Dev<R> d, Geo g, Prm<R> p, int nxblk) {
+__device__ __forceinline__ void trid_body(Dev<R> d, Geo g, Prm<R> p, int nxblk, const Blk &B) {
   const int nx = g.Ie - g.Ib + 1, ny = g.Je - g.Jb + 1;
-  if ((int)blockIdx.x < nxblk) {
-    int j = g.Jb + blockIdx.x * blockDim.x + threadIdx.x;
+  if ((int)B.x < nxblk) {
+    int j = g.Jb + B.x * blockDim.x + threadIdx.x;
     if (j > g.Je) return;
     thomas(d.xa, d.xc, d.xd, d.U, IX(g.Ib, j), 1, nx);
     return;
   }
-  int i = g.Ib + (blockIdx.x - nxblk) * blockDim.x + threadIdx.x;
+  int i = g.Ib + (B.x - nxblk) * blockDim.x + threadIdx.x;
   if (i > g.Ie) return;
   const ptrdiff_t s = (ptrdiff_t)g.P;
   if (!p.PERIODIC) {
@@ -1561,6 +1614,10 @@ __global__ void k_trid(Dev<R> d, Geo g, Prm<R> p, int nxblk) {
     size_t o = ob + k * s;
     d.V[o] = d.y1[o] + beta * d.y2[o];
   }
+}
+template <class R>
+__global__ void k_trid(Dev<R> d, Geo g, Prm<R> p, int nxblk) {
+  trid_body<R>(d, g, p, nxblk, grid_blk());
 }
 
 // Line solves staged through LDS.  A workgroup of 256 threads owns L lines
@@ -1639,10 +1696,10 @@ struct TridDIn {
 };
 
 template <class R, int L, int CH>
-__global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
+__device__ __forceinline__ void trid_lds_body(Dev<R> d, Geo g, int nxblk, const Blk &B) {
   __shared__ TridTile<R, L, CH> T;
-  const bool xdir = (int)blockIdx.x < nxblk;
-  const int blk = xdir ? blockIdx.x : blockIdx.x - nxblk;
+  const bool xdir = (int)B.x < nxblk;
+  const int blk = xdir ? B.x : B.x - nxblk;
   const int nlines_tot = xdir ? (g.Je - g.Jb + 1) : (g.Ie - g.Ib + 1);
   const int n = xdir ? (g.Ie - g.Ib + 1) : (g.Je - g.Jb + 1);
   const int line0 = blk * L;
@@ -1823,13 +1880,17 @@ __global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
   }
   if (w >= 2) bwd_store(0, 0);
 }
+template <class R, int L, int CH>
+__global__ void __launch_bounds__(256) k_trid_lds(Dev<R> d, Geo g, int nxblk) {
+  trid_lds_body<R, L, CH>(d, g, nxblk, grid_blk());
+}
 
 // GET_Eta_U_V_HU_HV tail (etauv_solver.F:568-593): dry cells, HU/HV, Froude
 // cap; without dispersion Ubar/Vbar := HU/HV on all cells.
 template <class R>
-__global__ void k_uv_final(Dev<R> d, Geo g, Prm<R> p) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;  // pointwise: launch order
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+__device__ __forceinline__ void uv_final_body(Dev<R> d, Geo g, Prm<R> p, const Blk &B) {
+  int i = 1 + B.x * blockDim.x + threadIdx.x;  // pointwise: launch order
+  int j = 1 + B.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
   if (i >= g.Ib && i <= g.Ie && j >= g.Jb && j <= g.Je) {
@@ -1856,11 +1917,19 @@ __global__ void k_uv_final(Dev<R> d, Geo g, Prm<R> p) {
   }
 }
 template <class R>
-__global__ void k_ubar_from_hu(Dev<R> d, size_t n) {
-  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+__global__ void k_uv_final(Dev<R> d, Geo g, Prm<R> p) {
+  uv_final_body<R>(d, g, p, grid_blk());
+}
+template <class R>
+__device__ __forceinline__ void ubar_from_hu_body(Dev<R> d, size_t n, const Blk &B) {
+  for (size_t q = (size_t)B.x * blockDim.x + threadIdx.x; q < n; q += (size_t)B.gx * blockDim.x) {
     d.Ubar[q] = d.HU[q];
     d.Vbar[q] = d.HV[q];
   }
+}
+template <class R>
+__global__ void k_ubar_from_hu(Dev<R> d, size_t n) {
+  ubar_from_hu_body<R>(d, n, grid_blk());
 }
 
 // --------------------------------------------------------------------------
@@ -1887,42 +1956,77 @@ __device__ __forceinline__ int new_mask(const Dev<R> &d, const Geo &g, int i, in
 }
 
 template <class R>
-__global__ void k_mask(Dev<R> d, Geo g, Prm<R> p) {
-  const Tile tb = xcd_tile();
+__device__ __forceinline__ void mask_body(Dev<R> d, Geo g, Prm<R> p, const Blk &B) {
+  const Tile tb = xcd_tile(B);
   int i = 1 + tb.x * blockDim.x + threadIdx.x;
   int j = 1 + tb.y * blockDim.y + threadIdx.y;
   if (i > g.M + 1 || j > g.N + 1) return;
   const int lo_i = g.Ib - 2, hi_i = g.Ie + 2, lo_j = g.Jb - 2, hi_j = g.Je + 2;
-  // x face (i, j): written by cells i-1 (-> Depth(i)) and i (-> Depth(i-1))
+  // Every value the thread may need is loaded first, from indices clamped
+  // into the array, and only then used where the reference's conditions
+  // take it (a store, or a load behind a branch, would hold back the loads
+  // after it): old MASK at the cell and its four neighbours, Eta at the
+  // same five, Depth at (i,j), (i-1,j), (i,j-1), MASK_STRUC.
+  const int ic = i <= g.M ? i : g.M, jc = j <= g.N ? j : g.N;
+  const int im = ic > 1 ? ic - 1 : 1, ip = ic < g.M ? ic + 1 : g.M;
+  const int jm = jc > 1 ? jc - 1 : 1, jp = jc < g.N ? jc + 1 : g.N;
+  const int im1 = i - 1 >= 1 ? i - 1 : 1, jm1 = j - 1 >= 1 ? j - 1 : 1;
+  const size_t o = IX(ic, jc);
+  const int m0 = d.MASK[o], mw = d.MASK[IX(im, jc)], me = d.MASK[IX(ip, jc)];
+  const int ms = d.MASK[IX(ic, jm)], mn = d.MASK[IX(ic, jp)];
+  const int mfw = d.MASK[IX(im1, jc)], mfs = d.MASK[IX(ic, jm1)];
+  const R e0 = d.Eta[o], ew = d.Eta[IX(im, jc)], ee = d.Eta[IX(ip, jc)];
+  const R es = d.Eta[IX(ic, jm)], en = d.Eta[IX(ic, jp)];
+  const R h0 = d.Depth[o], hw = d.Depth[IX(im1, jc)], hs = d.Depth[IX(ic, jm1)];
+  const int strc = d.MASK_STRUC[o];
+  // DepthX/DepthY truncation (masks.F:152-161): the x face (i, j) is
+  // written by cell i-1 (-> Depth(i)) and then by cell i (-> Depth(i-1)),
+  // the y face likewise; the later cell wins
   if (j <= g.N && j >= lo_j && j <= hi_j) {
     R v;
     bool w = false;
-    if (i - 1 >= lo_i && i - 1 <= hi_i && d.MASK[IX(i - 1, j)] < 1) { v = d.Depth[IX(i, j)]; w = true; }
-    if (i >= lo_i && i <= hi_i && d.MASK[IX(i, j)] < 1) { v = d.Depth[IX(i - 1, j)]; w = true; }
+    if (i - 1 >= lo_i && i - 1 <= hi_i && mfw < 1) { v = h0; w = true; }
+    if (i >= lo_i && i <= hi_i && m0 < 1) { v = hw; w = true; }
     if (w) d.DepthX[IX(i, j)] = v;
   }
   if (i <= g.M && i >= lo_i && i <= hi_i) {
     R v;
     bool w = false;
-    if (j - 1 >= lo_j && j - 1 <= hi_j && d.MASK[IX(i, j - 1)] < 1) { v = d.Depth[IX(i, j)]; w = true; }
-    if (j >= lo_j && j <= hi_j && d.MASK[IX(i, j)] < 1) { v = d.Depth[IX(i, j - 1)]; w = true; }
+    if (j - 1 >= lo_j && j - 1 <= hi_j && mfs < 1) { v = h0; w = true; }
+    if (j >= lo_j && j <= hi_j && m0 < 1) { v = hs; w = true; }
     if (w) d.DepthY[IX(i, j)] = v;
   }
   if (i > g.M || j > g.N) return;
-  size_t o = IX(i, j);
-  d.MASKn[o] = new_mask(d, g, i, j);
+  // the new mask of the cell (masks.F:79-135)
+  int m = m0;
+  if (i >= lo_i && i <= hi_i && j >= lo_j && j <= hi_j && strc == 1) {
+    if (m0 < 1) {
+      if (e0 > -h0) m = 1;
+      if (mw == 1 && ew > e0) m = 1;
+      if (me == 1 && ee > e0) m = 1;
+      if (ms == 1 && es > e0) m = 1;
+      if (mn == 1 && en > e0) m = 1;
+    } else {
+      if (e0 < -h0) m = 0;
+    }
+  }
+  d.MASKn[o] = m;
   int m9 = 0;
   if (i >= g.Ib - 1 && i <= g.Ie + 1 && j >= g.Jb - 1 && j <= g.Je + 1) {
     if (p.VISC_BRK) {
       m9 = 1;
     } else {
-      m9 = new_mask(d, g, i, j) * new_mask(d, g, i - 1, j) * new_mask(d, g, i + 1, j) * new_mask(d, g, i + 1, j + 1) *
+      m9 = m * new_mask(d, g, i - 1, j) * new_mask(d, g, i + 1, j) * new_mask(d, g, i + 1, j + 1) *
            new_mask(d, g, i, j + 1) * new_mask(d, g, i - 1, j + 1) * new_mask(d, g, i + 1, j - 1) *
            new_mask(d, g, i, j - 1) * new_mask(d, g, i - 1, j - 1);
-      if (ddiv(dabs(d.Eta[o]), rmax(d.Depth[o], p.MDF)) > p.SWE) m9 = 0;
+      if (ddiv(dabs(e0), rmax(h0, p.MDF)) > p.SWE) m9 = 0;
     }
   }
   d.MASK9[o] = m9;
+}
+template <class R>
+__global__ void k_mask(Dev<R> d, Geo g, Prm<R> p) {
+  mask_body<R>(d, g, p, grid_blk());
 }
 
 // --------------------------------------------------------------------------
@@ -2006,7 +2110,7 @@ __device__ __forceinline__ R brk_nu(const Dev<R> &d, const Geo &g, const Prm<R> 
 // routine of the step, formed only when every intermediate is kept.
 template <class R>
 __global__ void k_roller(Dev<R> d, Geo g, Prm<R> p) {
-  const Tile tb = xcd_tile();
+  const Tile tb = xcd_tile(grid_blk());
   const int i = 1 + tb.x * blockDim.x + threadIdx.x;
   const int j = 1 + tb.y * blockDim.y + threadIdx.y;
   if (i < g.Ib - 1 || i > g.Ie + 1 || j < g.Jb - 1 || j > g.Je + 1) return;
@@ -2032,8 +2136,8 @@ __global__ void k_roller(Dev<R> d, Geo g, Prm<R> p) {
 }
 
 template <class R>
-__global__ void k_brk1(Dev<R> d, Geo g, Prm<R> p, R DT) {
-  const Tile tb = xcd_tile();
+__device__ __forceinline__ void brk1_body(Dev<R> d, Geo g, Prm<R> p, R DT, const Blk &B) {
+  const Tile tb = xcd_tile(B);
   int i = 1 + tb.x * blockDim.x + threadIdx.x;
   int j = 1 + tb.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
@@ -2065,9 +2169,13 @@ __global__ void k_brk1(Dev<R> d, Geo g, Prm<R> p, R DT) {
   d.AGEn[o] = na;
   d.nu_break[o] = brk_nu(d, g, p, b, o, i, j, na);
 }
+template <class R>
+__global__ void k_brk1(Dev<R> d, Geo g, Prm<R> p, R DT) {
+  brk1_body<R>(d, g, p, DT, grid_blk());
+}
 
 template <class R>
-__global__ void __launch_bounds__(1024) k_brk2(Dev<R> d, Geo g, Prm<R> p, R DT) {
+__device__ __forceinline__ void brk2_body(Dev<R> d, Geo g, Prm<R> p, R DT, const Blk &B) {
   __shared__ int changed;
   const int n = *d.cand_n;
   if (n == 0) return;
@@ -2105,14 +2213,18 @@ __global__ void __launch_bounds__(1024) k_brk2(Dev<R> d, Geo g, Prm<R> p, R DT) 
   __syncthreads();
   if (threadIdx.x == 0) *d.cand_n = 0;
 }
+template <class R>
+__global__ void __launch_bounds__(1024) k_brk2(Dev<R> d, Geo g, Prm<R> p, R DT) {
+  brk2_body<R>(d, g, p, DT, grid_blk());
+}
 
 // --------------------------------------------------------------------------
 // EXCHANGE tail (bc.F:491-494): U,V,HU,HV *= MASK on all cells, optionally
 // followed by SPONGE_DAMPING (sponge.F:52-66).
 template <class R>
-__global__ void k_mask_mul(Dev<R> d, Geo g, Prm<R> p, int sponge, int mul) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;  // pointwise: launch order
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+__device__ __forceinline__ void mask_mul_body(Dev<R> d, Geo g, Prm<R> p, int sponge, int mul, const Blk &B) {
+  int i = 1 + B.x * blockDim.x + threadIdx.x;  // pointwise: launch order
+  int j = 1 + B.y * blockDim.y + threadIdx.y;
   if (i > g.M || j > g.N) return;
   size_t o = IX(i, j);
   // (MASK and SPONGE loaded together: one round trip)
@@ -2136,6 +2248,10 @@ __global__ void k_mask_mul(Dev<R> d, Geo g, Prm<R> p, int sponge, int mul) {
   }
   d.U[o] = u;
   d.V[o] = v;
+}
+template <class R>
+__global__ void k_mask_mul(Dev<R> d, Geo g, Prm<R> p, int sponge, int mul) {
+  mask_mul_body<R>(d, g, p, sponge, mul, grid_blk());
 }
 
 // FP64 budget (mass, energy, extrema) over the interior.

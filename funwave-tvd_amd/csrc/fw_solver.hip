@@ -10,91 +10,15 @@
 #include <string>
 #include <vector>
 
-#include "fw_kernels.h"
-#include "fwgpu.h"
+#include "fw_context.h"
 
 using namespace fw;
+using namespace fwi;
+
+
+
 
 namespace {
-
-const char *kFieldNames[F_COUNT] = {
-#define FW_NAME(n) #n,
-    FW_CELL_FIELDS(FW_NAME) FW_XFACE_FIELDS(FW_NAME) FW_YFACE_FIELDS(FW_NAME) FW_INT_FIELDS(FW_NAME)
-        FW_AUX_FIELDS(FW_NAME)
-#undef FW_NAME
-};
-
-FieldKind field_kind(int id) {
-  if (id < F_DepthX) return K_CELL;
-  if (id < F_DepthY) return K_XFACE;
-  if (id < F_MASK) return K_YFACE;
-  if (id < F_xmk_wk) return K_INT;
-  return K_AUX;
-}
-
-struct PendingEvent {
-  std::string name;
-  hipEvent_t a, b;
-};
-
-constexpr int kRedBlocks = 1024;
-constexpr int kStageSlots = 8;
-
-}  // namespace
-
-struct fw_ctx {
-  fw_config cfg{};
-  int rb = 4;
-  Geo g{};
-  hipStream_t stream = nullptr;
-  // fw_step runs UPDATE_MASK and WAVE_BREAKING on a second stream while the
-  // line solves run (they share no array; see do_src_huv); cur is the stream
-  // launch() uses
-  hipStream_t side = nullptr, cur = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  int device = 0;
-  std::vector<void *> allocs;
-  size_t bytes = 0;
-  Dev<float> df{};
-  Dev<double> dd{};
-  Prm<float> pf{};
-  Prm<double> pd{};
-  double dt = 0, time = 0;
-  // host copies of band tables
-  std::vector<double> xmk, ymk, omg;
-  bool wk_ready = false;
-  // pinned host ring for per-stage wavemaker factors and scalar readback
-  char *pinned = nullptr;
-  size_t slot_bytes = 0;
-  hipEvent_t slot_ev[kStageSlots]{};
-  int slot_next = 0;
-  void *scal = nullptr;  // device scalars
-  std::string err;
-  bool prof = false;
-  std::vector<PendingEvent> pending;
-  std::vector<hipEvent_t> evpool;
-  std::map<std::string, std::pair<int64_t, double>> acc;
-
-  template <class R> Dev<R> &dev();
-  template <class R> Prm<R> &prm();
-};
-template <> Dev<float> &fw_ctx::dev<float>() { return df; }
-template <> Dev<double> &fw_ctx::dev<double>() { return dd; }
-template <> Prm<float> &fw_ctx::prm<float>() { return pf; }
-template <> Prm<double> &fw_ctx::prm<double>() { return pd; }
-
-namespace {
-
-int fail(fw_ctx *c, int code, const std::string &msg) {
-  if (c) c->err = msg;
-  return code;
-}
-#define HIPCHK(c, x)                                                                  \
-  do {                                                                                \
-    hipError_t e_ = (x);                                                              \
-    if (e_ != hipSuccess)                                                             \
-      return fail(c, FW_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));      \
-  } while (0)
 
 hipEvent_t get_event(fw_ctx *c) {
   if (!c->evpool.empty()) {
@@ -199,7 +123,7 @@ void fill_prm(fw_ctx *c) {
   p.cyd = (R)2 * p.b2 / p.DY / p.DY;
 }
 
-size_t plane_elems(const Geo &g) { return (size_t)g.P * (size_t)(g.N + 1); }
+
 
 template <class R>
 int alloc_state(fw_ctx *c) {
@@ -212,6 +136,7 @@ int alloc_state(fw_ctx *c) {
     e = hipMemsetAsync(p, 0, bytes, c->stream);
     if (e != hipSuccess) return fail(c, FW_E_HIP, hipGetErrorString(e));
     c->allocs.push_back(p);
+    c->alloc_sizes.push_back(bytes);
     c->bytes += bytes;
     *out = p;
     return FW_OK;
@@ -256,27 +181,9 @@ int alloc_state(fw_ctx *c) {
   return FW_OK;
 }
 
-// ---- wavemaker tables (host, C library, working precision) ----------------
-template <class R> R hexp(R x);
-template <> float hexp(float x) { return expf(x); }
-template <> double hexp(double x) { return exp(x); }
-template <class R> R hsin(R x);
-template <> float hsin(float x) { return sinf(x); }
-template <> double hsin(double x) { return sin(x); }
-template <class R> R hcos(R x);
-template <> float hcos(float x) { return cosf(x); }
-template <> double hcos(double x) { return cos(x); }
-// COS and SIN of one argument (sources.F:153-154): the reference's compiler
-// evaluates such a pair with one sincos call, whose double result differs
-// from separate sin and cos calls of the C library in the last bit for
-// about 7 in 10^4 arguments, so the pair is taken from sincos here too
-template <class R> void hsincos(R x, R &s, R &c);
-template <> void hsincos(float x, float &s, float &c) { sincosf(x, &s, &c); }
-template <> void hsincos(double x, double &s, double &c) { sincos(x, &s, &c); }
-template <class R> R htanh(R x);
-template <> float htanh(float x) { return tanhf(x); }
-template <> double htanh(double x) { return tanh(x); }
-#define HPI ((R)3.141592653f)
+}  // namespace
+
+namespace fwi {
 
 template <class R>
 int prepare_wavemaker(fw_ctx *c) {
@@ -296,12 +203,40 @@ int prepare_wavemaker(fw_ctx *c) {
   return FW_OK;
 }
 
+template <class R>
+int wavemaker_factors(fw_ctx *c, R T, R &AA, R *h, size_t *n) {
+  const fw_config &f = c->cfg;
+  AA = (R)0;
+  *n = 0;
+  if (f.wavemaker == FW_WK_REG) {
+    AA = htanh<R>(HPI / ((R)f.Time_ramp * (R)f.Tperiod) * T) * (R)f.D_gen;
+    if ((int)c->ymk.size() < f.Nloc) return fail(c, FW_E_INVALID, "WK_REG needs ymk_wk uploaded");
+    for (int j = f.jlo; j <= f.jhi; j++)
+      h[j - f.jlo] = hsin<R>((R)f.rlamda * ((R)c->ymk[j - 1] - (R)0) - (R)2 * HPI / (R)f.Tperiod * T);
+    *n = f.jhi >= f.jlo ? (size_t)(f.jhi - f.jlo + 1) : 0;
+  } else if (f.wavemaker == FW_WK_DATA2D) {
+    AA = htanh<R>(HPI / ((R)f.Time_ramp / (R)f.FreqPeak) * T);
+    if ((int)c->omg.size() < f.NumFreq) return fail(c, FW_E_INVALID, "WK_DATA2D needs OMGN2D uploaded");
+    for (int k = 0; k < f.NumFreq; k++) hsincos<R>((R)c->omg[k] * T, h[f.NumFreq + k], h[k]);
+    *n = (size_t)(2 * f.NumFreq);
+  }
+  return FW_OK;
+}
+
+template int prepare_wavemaker<float>(fw_ctx *);
+template int prepare_wavemaker<double>(fw_ctx *);
+template int wavemaker_factors<float>(fw_ctx *, float, float &, float *, size_t *);
+template int wavemaker_factors<double>(fw_ctx *, double, double &, double *, size_t *);
+
+}  // namespace fwi
+
+namespace {
+
 // per-stage factors: returns AA; queues the copy of the row/frequency table
 template <class R>
 int stage_wavemaker(fw_ctx *c, R &AA) {
-  const fw_config &f = c->cfg;
   AA = (R)0;
-  if (f.wavemaker == FW_WK_NONE) return FW_OK;
+  if (c->cfg.wavemaker == FW_WK_NONE) return FW_OK;
   int rc = prepare_wavemaker<R>(c);
   if (rc) return rc;
   const R T = (R)c->time;
@@ -310,20 +245,7 @@ int stage_wavemaker(fw_ctx *c, R &AA) {
   HIPCHK(c, hipEventSynchronize(c->slot_ev[slot]));
   R *h = (R *)(c->pinned + (size_t)slot * c->slot_bytes);
   size_t n = 0;
-  if (f.wavemaker == FW_WK_REG) {
-    AA = htanh<R>(HPI / ((R)f.Time_ramp * (R)f.Tperiod) * T) * (R)f.D_gen;
-    if ((int)c->ymk.size() < f.Nloc) return fail(c, FW_E_INVALID, "WK_REG needs ymk_wk uploaded");
-    for (int j = f.jlo; j <= f.jhi; j++)
-      h[j - f.jlo] = hsin<R>((R)f.rlamda * ((R)c->ymk[j - 1] - (R)0) - (R)2 * HPI / (R)f.Tperiod * T);
-    n = (size_t)(f.jhi - f.jlo + 1);
-  } else if (f.wavemaker == FW_WK_DATA2D) {
-    AA = htanh<R>(HPI / ((R)f.Time_ramp / (R)f.FreqPeak) * T);
-    if ((int)c->omg.size() < f.NumFreq) return fail(c, FW_E_INVALID, "WK_DATA2D needs OMGN2D uploaded");
-    for (int k = 0; k < f.NumFreq; k++) {
-      hsincos<R>((R)c->omg[k] * T, h[f.NumFreq + k], h[k]);
-    }
-    n = (size_t)(2 * f.NumFreq);
-  }
+  if ((rc = wavemaker_factors<R>(c, T, AA, h, &n))) return rc;
   HIPCHK(c, hipMemcpyAsync(c->dev<R>().wk_stage, h, n * sizeof(R), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipEventRecord(c->slot_ev[slot], c->stream));
   return FW_OK;
@@ -347,9 +269,6 @@ void halo_exchange_list(fw_ctx *c, HaloList &L, HaloList &Lf) {
   }
 }
 
-inline void hadd(HaloList &L, void *p, int vtype, int isint = 0, int spec = 0) {
-  L.a[L.n++] = HaloArr{p, vtype, isint, spec};
-}
 
 // EXCHANGE (bc.F:426-529), optionally followed by SPONGE_DAMPING
 template <class R>
@@ -610,6 +529,18 @@ int do_roller(fw_ctx *c) {
   return FW_OK;
 }
 
+// The breaking kernels raise flags[0] when the in-order age propagation
+// does not settle.  Once the host has seen it, it is cleared, so that the
+// error is reported for the call that raised it and not for every later one.
+int read_break_flag(fw_ctx *c, void *flags) {
+  int flag = 0;
+  HIPCHK(c, hipMemcpy(&flag, flags, sizeof(int), hipMemcpyDeviceToHost));
+  if (!flag) return FW_OK;
+  const int zero = 0;
+  HIPCHK(c, hipMemcpy(flags, &zero, sizeof(int), hipMemcpyHostToDevice));
+  return fail(c, FW_E_BREAK_ITER, "breaking-age propagation did not settle");
+}
+
 template <class R>
 int do_step(fw_ctx *c, int nsteps) {
   Dev<R> &d = c->dev<R>();
@@ -629,9 +560,7 @@ int do_step(fw_ctx *c, int nsteps) {
     }
     double m;
     if ((rc = do_max_eta<R>(c, &m))) return rc;
-    int flag = 0;
-    HIPCHK(c, hipMemcpy(&flag, d.flags, sizeof(int), hipMemcpyDeviceToHost));
-    if (flag) return fail(c, FW_E_BREAK_ITER, "breaking-age propagation did not settle");
+    if ((rc = read_break_flag(c, (void *)d.flags))) return rc;
     if (m > f.EtaBlowVal) {
       char buf[128];
       snprintf(buf, sizeof buf, "blow-up: max|eta|=%g > EtaBlowVal=%g at time %g", m, f.EtaBlowVal, c->time);
@@ -944,10 +873,7 @@ int fw_wave_breaking(fw_ctx *c) {
   int rc = DISPATCH(c, do_breaking)(c);
   if (rc == FW_OK) rc = DISPATCH(c, do_roller)(c);
   if (rc) return rc;
-  int flag = 0;
-  void *fl = c->rb == 8 ? (void *)c->dd.flags : (void *)c->df.flags;
-  HIPCHK(c, hipMemcpy(&flag, fl, sizeof(int), hipMemcpyDeviceToHost));
-  if (flag) return fail(c, FW_E_BREAK_ITER, "breaking-age propagation did not settle");
+  if ((rc = read_break_flag(c, c->rb == 8 ? (void *)c->dd.flags : (void *)c->df.flags))) return rc;
   return c->prof ? collect_profile(c) : FW_OK;
 }
 int fw_sponge_damping(fw_ctx *c) { ENTRY(c, DISPATCH(c, do_sponge_only)(c)); }

@@ -41,7 +41,10 @@ EXPORTS = ["fw_create", "fw_destroy", "fw_strerror", "fw_synchronize",
            "fw_sponge_damping", "fw_check_blowup", "fw_step", "fw_reduce_stats",
            "fw_set_option", "fw_profile_enable", "fw_profile_reset", "fw_profile_get",
            "fw_profile_names", "fw_time_steps", "fw_field_device", "fw_get_stream",
-           "fw_get_config"]
+           "fw_get_config", "fw_batch_create", "fw_batch_destroy", "fw_batch_strerror",
+           "fw_batch_size", "fw_batch_step", "fw_batch_status", "fw_batch_time_steps",
+           "fw_batch_set_graph", "fw_batch_autotune", "fw_batch_get_shapes"]
+FW_BATCH_RUNNING, FW_BATCH_BLOWUP, FW_BATCH_DT_CHANGED, FW_BATCH_BREAK_ITER = 0, 1, 2, 3
 
 
 class Config(ct.Structure):
@@ -99,6 +102,17 @@ def load_library(path: str = LIB_PATH):
     lib.fw_profile_get.argtypes = [vp, ct.c_char_p, ct.POINTER(ct.c_int64), dp]
     lib.fw_profile_names.argtypes = [vp, ct.c_char_p, ip]
     lib.fw_time_steps.argtypes = [vp, ip, dp]
+    lib.fw_batch_create.argtypes = [ct.POINTER(vp), ip, ct.POINTER(vp)]
+    lib.fw_batch_destroy.argtypes = [vp]
+    lib.fw_batch_strerror.argtypes = [vp]
+    lib.fw_batch_strerror.restype = ct.c_char_p
+    lib.fw_batch_size.argtypes = [vp, ct.POINTER(ip)]
+    lib.fw_batch_step.argtypes = [vp, ip]
+    lib.fw_batch_status.argtypes = [vp, ct.POINTER(ip)]
+    lib.fw_batch_time_steps.argtypes = [vp, ip, dp]
+    lib.fw_batch_set_graph.argtypes = [vp, ip]
+    lib.fw_batch_autotune.argtypes = [vp, ip, ct.c_char_p, ip]
+    lib.fw_batch_get_shapes.argtypes = [vp, ct.POINTER(ip)]
     _LIB = lib
     return lib
 
@@ -318,3 +332,62 @@ YFACE = _names("FW_YFACE_FIELDS")
 INTF = _names("FW_INT_FIELDS")
 AUX = _names("FW_AUX_FIELDS")
 ALL_FIELDS = CELL + XFACE + YFACE + INTF + AUX
+
+
+class Batch:
+    """Ensemble of Solver members advanced together (fw_batch_*): one launch
+    per phase of the step over all members, one captured graph per step.
+    Members need FIXED_DT; they stay usable (download, step, ...) and must
+    outlive the batch."""
+
+    def __init__(self, members):
+        self.lib = load_library()
+        self.members = list(members)
+        arr = (ct.c_void_p * len(self.members))(*[m.ctx.value for m in self.members])
+        self.b = ct.c_void_p()
+        rc = self.lib.fw_batch_create(arr, len(self.members), ct.byref(self.b))
+        if rc != FW_OK:
+            raise FwError(rc, "fw_batch_create: " + self.members[0].lib.fw_strerror(self.members[0].ctx).decode())
+
+    def _check(self, rc):
+        if rc != FW_OK:
+            raise FwError(rc, self.lib.fw_batch_strerror(self.b).decode())
+        return rc
+
+    def step(self, n=1):
+        self._check(self.lib.fw_batch_step(self.b, n))
+
+    def time_steps(self, n):
+        ms = ct.c_double()
+        self._check(self.lib.fw_batch_time_steps(self.b, n, ct.byref(ms)))
+        return ms.value
+
+    def status(self):
+        st = (ct.c_int * len(self.members))()
+        self._check(self.lib.fw_batch_status(self.b, st))
+        return list(st)
+
+    def set_graph(self, on=True):
+        self._check(self.lib.fw_batch_set_graph(self.b, 1 if on else 0))
+
+    def autotune(self, trial_steps=2):
+        buf = ct.create_string_buffer(8192)
+        self._check(self.lib.fw_batch_autotune(self.b, trial_steps, buf, 8192))
+        return buf.value.decode()
+
+    def shapes(self):
+        s = (ct.c_int * 8)()
+        self._check(self.lib.fw_batch_get_shapes(self.b, s))
+        names = ("disp", "src_huv", "cell", "mask_brk")
+        return {k: (s[2 * i], s[2 * i + 1]) for i, k in enumerate(names)}
+
+    def close(self):
+        if self.b:
+            self.lib.fw_batch_destroy(self.b)
+            self.b = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -154,6 +154,51 @@ int fw_profile_names(fw_ctx *ctx, char *buf, int buflen);
 /* device milliseconds of nsteps whole steps, HIP events on the ctx stream */
 int fw_time_steps(fw_ctx *ctx, int nsteps, double *ms);
 
+/* --- ensembles -------------------------------------------------------------
+ * No counterpart in the reference, whose main loop advances one domain
+ * (src/main.F:284-535).  For callers running many independent domains
+ * ("members", each a context created and uploaded as above, of any size):
+ * every phase of the step is one launch over all members, whose workgroups
+ * take their member and tile from per-member tile queues; each member's DT,
+ * TIME, blow-up check and the wavemaker factors of the whole run stay on
+ * the device, and one step replays from a captured hipGraph.  Results are
+ * bitwise those of fw_step on each member alone.
+ *
+ * Members need FIXED_DT (the per-step wavemaker factors are evaluated ahead
+ * on the host, with the C library, for TIME = TIME + DT_fixed step after
+ * step); a member whose time step would halve (misc.F:283-288) stops before
+ * that step with status FW_BATCH_DT_CHANGED, one whose max |eta| exceeds
+ * EtaBlowVal stops after its step with FW_BATCH_BLOWUP; the others go on.
+ * Members share the precision, DISPERSION, PERIODIC and whether DX and DY
+ * are powers of two, and run with FW_OPT_FULL_STATE = 0.  The batch keeps
+ * pointers to the contexts: they must outlive it, and per-member calls on
+ * them may be interleaved with batch calls (each batch call picks up the
+ * members' current state, and afterwards fw_get_clock, fw_download, ...
+ * see the advanced state).  Calls on one batch come from one host thread. */
+typedef struct fw_batch fw_batch;
+enum { FW_BATCH_RUNNING = 0, FW_BATCH_BLOWUP = 1, FW_BATCH_DT_CHANGED = 2, FW_BATCH_BREAK_ITER = 3 };
+int fw_batch_create(fw_ctx *const *members, int n, fw_batch **out);
+int fw_batch_destroy(fw_batch *b);
+const char *fw_batch_strerror(const fw_batch *b);
+int fw_batch_size(const fw_batch *b, int *n);
+/* nsteps whole steps (main.F:295-517) of every running member.  Returns 0,
+ * or for the first member that stopped: FW_E_BLOWUP, FW_E_INVALID (time
+ * step would change) or FW_E_BREAK_ITER; fw_batch_status tells which. */
+int fw_batch_step(fw_batch *b, int nsteps);
+int fw_batch_status(const fw_batch *b, int *status /* n entries */);
+/* as fw_batch_step, and the device milliseconds of the steps (HIP events
+ * around the graph launches on the batch's stream) */
+int fw_batch_time_steps(fw_batch *b, int nsteps, double *ms);
+/* 1 (default): replay the captured step graphs; 0: launch eagerly */
+int fw_batch_set_graph(fw_batch *b, int on);
+/* Workgroup shapes of the tunable kernels (the dispersion stencils, the RK
+ * update, the velocity tail and mask multiply, UPDATE_MASK and breaking)
+ * chosen by timing trial_steps steps per candidate on a snapshot of the
+ * members' state, restored afterwards; report: one line per trial. */
+int fw_batch_autotune(fw_batch *b, int trial_steps, char *report, int len);
+/* the shapes in use: 4 groups x (x, y) */
+int fw_batch_get_shapes(const fw_batch *b, int *shapes);
+
 #ifdef __cplusplus
 }
 #endif
