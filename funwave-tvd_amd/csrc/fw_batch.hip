@@ -163,6 +163,12 @@ __global__ void __launch_bounds__(WideLB<R>::wg512) kb_disp_terms(FWB_ARGS, int 
   disp_terms_body<R, P2>(mdev(M, pm, pa), M.g, M.p, M.clk[0], tile_blk(e));
 }
 
+template <class R, bool P2>
+__global__ void __launch_bounds__(kDispTX * kDispTY) kb_disp_fused(FWB_ARGS, int pm, int pa) {
+  FWB_HEAD
+  disp_fused_body<R, P2>(mdev(M, pm, pa), M.g, M.p, M.clk[0], tile_blk(e));
+}
+
 template <class R, bool DISP>
 __global__ void __launch_bounds__(256) kb_flux_x(FWB_ARGS, int pm, int pa) {
   FWB_HEAD
@@ -267,7 +273,8 @@ __global__ void kb_reset_steps(int *const *st, int n) {
 // tunable workgroup shapes (the kernels take any shape within their
 // launch bounds); names as reported by fw_batch_autotune
 enum Group { G_DISP = 0, G_HUV, G_CELL, G_BRK, G_COUNT };
-const char *kGroupName[G_COUNT] = {"disp (disp_deriv, disp_terms)", "src_huv", "cell (uv_final, mask_mul)",
+const char *kGroupName[G_COUNT] = {"disp (disp_deriv, disp_terms; periodic members)", "src_huv",
+                                   "cell (uv_final, mask_mul)",
                                    "mask/brk (mask, brk1)"};
 
 struct Queue {
@@ -491,6 +498,7 @@ int build_queues(fw_batch *b) {
   const dim3 sd = shape_or(b->shape[G_DISP], dim3(64, 8)), sh = shape_or(b->shape[G_HUV], dim3(64, 4));
   const dim3 sc = shape_or(b->shape[G_CELL], dim3(64, 4)), sb = shape_or(b->shape[G_BRK], dim3(64, 8));
   if ((rc = make_tiles(b, "disp", sd, sd.x, sd.y, M_, N_))) return rc;
+  if ((rc = make_tiles(b, "dispf", dim3(kDispTX, kDispTY), kDispTX, kDispTY, M_, N_))) return rc;
   if ((rc = make_tiles(b, "huv", sh, sh.x, sh.y, M_, N_))) return rc;
   if ((rc = make_tiles(b, "cell", sc, sc.x, sc.y, M_, N_))) return rc;
   if ((rc = make_tiles(b, "brk1", sb, sb.x, sb.y, M_, N_))) return rc;
@@ -567,7 +575,9 @@ int rec_step(fw_batch *b, int p0) {
   bl<R>(b, s, "dt", kb_dt_partial<R>, par, par);
   bl<R>(b, s, "one", kb_dt_final<R>);
   for (int st = 0; st < 3; st++) {
-    if (b->disp) {
+    if (b->disp && !b->periodic) {
+      bl<R>(b, s, "dispf", kb_disp_fused<R, P2>, par, par);
+    } else if (b->disp) {
       bl<R>(b, s, "disp", kb_disp_deriv<R, P2>, par, par);
       if (!b->periodic) {
         bl<R>(b, s, "halo_d", kb_halo<R>, 2, 0, par);
@@ -858,6 +868,9 @@ int autotune(fw_batch *b, int trial, std::string &rep) {
     return FW_OK;
   };
   for (int gi = 0; gi < G_COUNT && rc == FW_OK; gi++) {
+    // the two-pass dispersion kernels run only for periodic members (the
+    // others take the fused kernel, whose tile is fixed)
+    if (gi == G_DISP && !(b->disp && b->periodic)) continue;
     dim3 best = cand[gi][0];
     double best_ms = 1e30;
     for (const dim3 &s : cand[gi]) {

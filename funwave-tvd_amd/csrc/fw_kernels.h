@@ -484,12 +484,12 @@ __global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_deriv(Dev<R> d, Geo g
 
 // CAL_DISPERSION part 2: U4, V4, U1p, V1p on all cells (:293-330) and the
 // nonlinear terms U1pp..V3 on the interior (:398-503).
-template <class R, bool P2>
-__device__ __forceinline__ void disp_terms_body(Dev<R> d, Geo g, Prm<R> p, R DT, const Blk &B) {
-  const Tile tb = xcd_tile(B);
-  int i = 1 + tb.x * blockDim.x + threadIdx.x;
-  int j = 1 + tb.y * blockDim.y + threadIdx.y;
-  if (i > g.M || j > g.N) return;
+// The terms of cell (i, j).  SD(k, di, dj) gives second derivative k (0
+// Uxx, 1 Vxy, 2 Uxy, 3 Vyy, 4 DUxx, 5 DVxy, 6 DUxy, 7 DVyy) at (i+di, j+dj)
+// after their ghost fill, and 8, 9, 10 give ETAx, ETAy, ETAT at (i, j): from
+// the arrays (k_disp_terms) or from the fused kernel's tile (k_disp_fused).
+template <class R, bool P2, class SD>
+__device__ __forceinline__ void disp_terms_at(Dev<R> &d, const Geo &g, const Prm<R> &p, R DT, int i, int j, SD S) {
   size_t o = IX(i, j);
   const R B1 = p.B1, B2 = p.B2, b2 = p.b2, g2 = p.G2;
   const Dv<R> dx = dvx<P2>(p), dy = dvy<P2>(p);
@@ -497,8 +497,8 @@ __device__ __forceinline__ void disp_terms_body(Dev<R> d, Geo g, Prm<R> p, R DT,
   R r_u4, r_v4, r_u1p, r_v1p;
   {
     R dep = d.Depth[o];
-    R uxx = d.Uxx[o], vxy = d.Vxy[o], uxy = d.Uxy[o], vyy = d.Vyy[o];
-    R duxx = d.DUxx[o], dvxy = d.DVxy[o], duxy = d.DUxy[o], dvyy = d.DVyy[o];
+    R uxx = S(0, 0, 0), vxy = S(1, 0, 0), uxy = S(2, 0, 0), vyy = S(3, 0, 0);
+    R duxx = S(4, 0, 0), dvxy = S(5, 0, 0), duxy = S(6, 0, 0), dvyy = S(7, 0, 0);
     R k0 = c13 - B1 + (R)0.5 * B1 * B1;
     R u4 = k0 * dep * dep * (uxx + vxy) + (B1 - c12) * dep * (duxx + dvxy);
     R v4 = k0 * dep * dep * (uxy + vyy) + (B1 - c12) * dep * (duxy + dvyy);
@@ -558,23 +558,23 @@ __device__ __forceinline__ void disp_terms_body(Dev<R> d, Geo g, Prm<R> p, R DT,
   const R f_DUtxy = wall ? (R)0 : gfxy<R>(DUT, i, j, dx, dy) * m9;
   const R f_DVtxy = wall ? (R)0 : gfxy<R>(DVT, i, j, dx, dy) * m9;
 #define A_(n, a, b) d.n[IX(a, b)]
-  R UxxVxy = A_(Uxx, i, j) + A_(Vxy, i, j);
-  R UxyVyy = A_(Uxy, i, j) + A_(Vyy, i, j);
-  R UxxVxy_x = dv(half(A_(Uxx, i + 1, j) + A_(Vxy, i + 1, j) - A_(Uxx, i - 1, j) - A_(Vxy, i - 1, j)), dx);
-  R UxxVxy_y = dv(half(A_(Uxx, i, j + 1) + A_(Vxy, i, j + 1) - A_(Uxx, i, j - 1) - A_(Vxy, i, j - 1)), dy);
-  R UxyVyy_x = dv(half(A_(Uxy, i + 1, j) + A_(Vyy, i + 1, j) - A_(Uxy, i - 1, j) - A_(Vyy, i - 1, j)), dx);
-  R UxyVyy_y = dv(half(A_(Uxy, i, j + 1) + A_(Vyy, i, j + 1) - A_(Uxy, i, j - 1) - A_(Vyy, i, j - 1)), dy);
-  R HUxxHVxy = A_(DUxx, i, j) + A_(DVxy, i, j);
-  R HUxyHVyy = A_(DUxy, i, j) + A_(DVyy, i, j);
-  R HUxxHVxy_x = dv(half(A_(DUxx, i + 1, j) + A_(DVxy, i + 1, j) - A_(DUxx, i - 1, j) - A_(DVxy, i - 1, j)), dx);
-  R HUxxHVxy_y = dv(half(A_(DUxx, i, j + 1) + A_(DVxy, i, j + 1) - A_(DUxx, i, j - 1) - A_(DVxy, i, j - 1)), dy);
-  R HUxyHVyy_x = dv(half(A_(DUxy, i + 1, j) + A_(DVyy, i + 1, j) - A_(DUxy, i - 1, j) - A_(DVyy, i - 1, j)), dx);
-  R HUxyHVyy_y = dv(half(A_(DUxy, i, j + 1) + A_(DVyy, i, j + 1) - A_(DUxy, i, j - 1) - A_(DVyy, i, j - 1)), dy);
+  R UxxVxy = S(0, 0, 0) + S(1, 0, 0);
+  R UxyVyy = S(2, 0, 0) + S(3, 0, 0);
+  R UxxVxy_x = dv(half(S(0, 1, 0) + S(1, 1, 0) - S(0, -1, 0) - S(1, -1, 0)), dx);
+  R UxxVxy_y = dv(half(S(0, 0, 1) + S(1, 0, 1) - S(0, 0, -1) - S(1, 0, -1)), dy);
+  R UxyVyy_x = dv(half(S(2, 1, 0) + S(3, 1, 0) - S(2, -1, 0) - S(3, -1, 0)), dx);
+  R UxyVyy_y = dv(half(S(2, 0, 1) + S(3, 0, 1) - S(2, 0, -1) - S(3, 0, -1)), dy);
+  R HUxxHVxy = S(4, 0, 0) + S(5, 0, 0);
+  R HUxyHVyy = S(6, 0, 0) + S(7, 0, 0);
+  R HUxxHVxy_x = dv(half(S(4, 1, 0) + S(5, 1, 0) - S(4, -1, 0) - S(5, -1, 0)), dx);
+  R HUxxHVxy_y = dv(half(S(4, 0, 1) + S(5, 0, 1) - S(4, 0, -1) - S(5, 0, -1)), dy);
+  R HUxyHVyy_x = dv(half(S(6, 1, 0) + S(7, 1, 0) - S(6, -1, 0) - S(7, -1, 0)), dx);
+  R HUxyHVyy_y = dv(half(S(6, 0, 1) + S(7, 0, 1) - S(6, 0, -1) - S(7, 0, -1)), dy);
   R rh = A_(Depth, i, j);
   R rhx = dv(half(A_(Depth, i + 1, j) - A_(Depth, i - 1, j)), dx);
   R rhy = dv(half(A_(Depth, i, j + 1) - A_(Depth, i, j - 1)), dy);
   R reta = A_(Eta, i, j);
-  R ex = A_(ETAx, i, j), ey = A_(ETAy, i, j), et = A_(ETAT, i, j);
+  R ex = S(8, 0, 0), ey = S(9, 0, 0), et = S(10, 0, 0);
   R Utx_ = f_Utx, Vty_ = f_Vty;
   R u1pp = -reta * B2 * ex * B2 * (Utx_ + Vty_) - (R)0.5 * reta * reta * B2 * B2 * (f_Utxx + f_Vtxy)
            - ex * B2 * (f_DUtx + f_DVty) - reta * B2 * (f_DUtxx + f_DVtxy);
@@ -623,8 +623,127 @@ __device__ __forceinline__ void disp_terms_body(Dev<R> d, Geo g, Prm<R> p, R DT,
 #undef A_
 }
 template <class R, bool P2>
+__device__ __forceinline__ void disp_terms_body(Dev<R> d, Geo g, Prm<R> p, R DT, const Blk &B) {
+  const Tile tb = xcd_tile(B);
+  int i = 1 + tb.x * blockDim.x + threadIdx.x;
+  int j = 1 + tb.y * blockDim.y + threadIdx.y;
+  if (i > g.M || j > g.N) return;
+  R *const arr[11] = {d.Uxx, d.Vxy, d.Uxy, d.Vyy, d.DUxx, d.DVxy, d.DUxy, d.DVyy, d.ETAx, d.ETAy, d.ETAT};
+  auto S = [&](int k, int di, int dj) { return arr[k][IX(i + di, j + dj)]; };
+  disp_terms_at<R, P2>(d, g, p, DT, i, j, S);
+}
+template <class R, bool P2>
 __global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_terms(Dev<R> d, Geo g, Prm<R> p, R DT) {
   disp_terms_body<R, P2>(d, g, p, DT, grid_blk());
+}
+
+// --------------------------------------------------------------------------
+// CAL_DISPERSION in one pass (dispersion.F:62-506), for the production path
+// of a non-periodic domain: the eight second derivatives of a 64 x 8 tile and
+// its one-cell ring go to LDS, already as EXCHANGE_DISPERSION leaves them
+// (a ghost cell holds the value of its mirror cell, negated for the arrays
+// odd in that direction, bc.F:306-529), and the terms of k_disp_terms are
+// formed from there.  The derivatives are not written to HBM and read back:
+// only what later routines read is stored (ETAT, ETAx, ETAy, the four cross
+// derivatives, U4, V4, U1p, V1p and the nonlinear sums).  Same operations on
+// the same operands as k_disp_deriv + ghost fill + k_disp_terms.
+constexpr int kDispTX = 64, kDispTY = 8;
+
+// the eight second derivatives at (i, j) after the ghost fill (0 Uxx, 1
+// Vxy, 2 Uxy, 3 Vyy, 4 DUxx, 5 DVxy, 6 DUxy, 7 DVyy)
+template <class R, bool P2>
+__device__ __forceinline__ void second_derivs(const Dev<R> &d, const Geo &g, const Prm<R> &p, int i, int j, R *v) {
+  const Dv<R> dx = dvx<P2>(p), dy = dvy<P2>(p);
+  const R mdf = p.MDF;
+  int si = i, sj = j;
+  bool nx = false, ny = false;
+  if (i < g.Ib) { si = 2 * g.Ib - 1 - i; nx = true; }
+  if (i > g.Ie) { si = 2 * g.Ie + 1 - i; nx = true; }
+  if (j < g.Jb) { sj = 2 * g.Jb - 1 - j; ny = true; }
+  if (j > g.Je) { sj = 2 * g.Je + 1 - j; ny = true; }
+  const R m9 = (R)d.MASK9[IX(si, sj)];
+  const R *U = d.U, *V = d.V, *Dp = d.Depth;
+  auto DU = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * U[q]; };
+  auto DV = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * V[q]; };
+  const bool wall = (si == g.Ib || si == g.Ie || sj == g.Jb || sj == g.Je);
+  const R uxx = dfxx(g, U, si, sj, dx) * m9;
+  const R uxy_s = dfxy(g, U, si, sj, dx, dy) * m9, vxy_s = dfxy(g, V, si, sj, dx, dy) * m9;
+  const R vyy = dfyy(g, V, si, sj, dy) * m9;
+  const R duxx = gfxx<R>(DU, si, sj, dx) * m9;
+  const R duxy_s = gfxy<R>(DU, si, sj, dx, dy) * m9, dvxy_s = gfxy<R>(DV, si, sj, dx, dy) * m9;
+  const R dvyy = gfyy<R>(DV, si, sj, dy) * m9;
+  v[0] = nx ? -uxx : uxx;
+  v[1] = wall ? (R)0 : vxy_s;
+  v[2] = wall ? (R)0 : uxy_s;
+  v[3] = ny ? -vyy : vyy;
+  v[4] = nx ? -duxx : duxx;
+  v[5] = wall ? (R)0 : dvxy_s;
+  v[6] = wall ? (R)0 : duxy_s;
+  v[7] = ny ? -dvyy : dvyy;
+}
+
+template <class R, bool P2>
+__device__ __forceinline__ void disp_fused_body(Dev<R> d, Geo g, Prm<R> p, R DT, const Blk &B) {
+  constexpr int EX = kDispTX + 2, EY = kDispTY + 2;
+  __shared__ R sS[8][EY][EX];
+  const Tile tb = xcd_tile(B);
+  const int x0 = 1 + tb.x * kDispTX, y0 = 1 + tb.y * kDispTY;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  for (int q = tx + ty * kDispTX; q < EX * EY; q += kDispTX * kDispTY) {
+    const int ex = q % EX, ey = q / EX;
+    const int i = x0 - 1 + ex, j = y0 - 1 + ey;
+    R v[8];
+    if (i >= 1 && i <= g.M && j >= 1 && j <= g.N) {
+      second_derivs<R, P2>(d, g, p, i, j, v);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; k++) v[k] = (R)0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) sS[k][ey][ex] = v[k];
+  }
+  __syncthreads();
+  const int i = x0 + tx, j = y0 + ty;
+  if (i > g.M || j > g.N) return;
+  const size_t o = IX(i, j);
+  const Dv<R> dx = dvx<P2>(p), dy = dvy<P2>(p);
+  const bool g2 = p.G2 > (R)0;
+  const bool inner = i >= g.Ib && i <= g.Ie && j >= g.Jb && j <= g.Je;
+  // ETAT (dispersion.F:103-119) on 1..Mloc-1 x 1..Nloc-1
+  const bool has_etat = i <= g.M - 1 && j <= g.N - 1;
+  R etat = (R)0;
+  if (has_etat) etat = -dv(d.P[IX(i + 1, j)] - d.P[IX(i, j)], dx) - dv(d.Q[IX(i, j + 1)] - d.Q[IX(i, j)], dy);
+  // ETAx, ETAy on the interior; their ghost fill with the other arrays
+  // when Gamma2 > 0
+  int si = i, sj = j;
+  bool nx = false, ny = false;
+  if (i < g.Ib) { si = 2 * g.Ib - 1 - i; nx = true; }
+  if (i > g.Ie) { si = 2 * g.Ie + 1 - i; nx = true; }
+  if (j < g.Jb) { sj = 2 * g.Jb - 1 - j; ny = true; }
+  if (j > g.Je) { sj = 2 * g.Je + 1 - j; ny = true; }
+  const R m9s = (R)d.MASK9[IX(si, sj)];
+  R etax = dfx(g, d.Eta, si, sj, dx) * m9s, etay = dfy(g, d.Eta, si, sj, dy) * m9s;
+  if (nx) etax = -etax;
+  if (ny) etay = -etay;
+  const int lx = tx + 1, ly = ty + 1;
+  auto S = [&](int k, int di, int dj) -> R {
+    if (k == 8) return etax;
+    if (k == 9) return etay;
+    if (k == 10) return etat;
+    return sS[k][ly + dj][lx + di];
+  };
+  const R uxy = sS[2][ly][lx], vxy = sS[1][ly][lx], duxy = sS[6][ly][lx], dvxy = sS[5][ly][lx];
+  disp_terms_at<R, P2>(d, g, p, DT, i, j, S);
+  if (has_etat) d.ETAT[o] = etat;
+  d.Uxy[o] = uxy; d.Vxy[o] = vxy; d.DUxy[o] = duxy; d.DVxy[o] = dvxy;
+  if (inner ? (g2 || p.SHOW_BRK) : g2) {
+    d.ETAx[o] = etax;
+    d.ETAy[o] = etay;
+  }
+}
+template <class R, bool P2>
+__global__ void __launch_bounds__(kDispTX * kDispTY) k_disp_fused(Dev<R> d, Geo g, Prm<R> p, R DT) {
+  disp_fused_body<R, P2>(d, g, p, DT, grid_blk());
 }
 
 // --------------------------------------------------------------------------
@@ -2255,28 +2374,30 @@ __global__ void k_mask_mul(Dev<R> d, Geo g, Prm<R> p, int sponge, int mul) {
 }
 
 // FP64 budget (mass, energy, extrema) over the interior.
-template <class R>
-__global__ void k_stats(Dev<R> d, Geo g, Prm<R> p) {
-  __shared__ double sm[7][256];
+// FP64 budget terms of one wet interior cell (fw_stats): total depth,
+// volume and energy formed in FP64
+struct StatAcc {
   double mass = 0, en = 0, mx = -1e300, mn = 1e300, mu = 0, mv = 0, wet = 0;
-  int nx = g.Ie - g.Ib + 1, n = nx * (g.Je - g.Jb + 1);
-  double dxdy = (double)p.DX * (double)p.DY;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    int i = g.Ib + q % nx, j = g.Jb + q / nx;
-    size_t o = IX(i, j);
-    if (d.MASK[o] < 1) continue;
-    wet += 1.0;
-    double e = (double)d.Eta[o], u = (double)d.U[o], v = (double)d.V[o];
-    double h = e + (double)d.Depth[o];  // total depth formed in FP64
-    mass += e * dxdy;
-    en += (0.5 * h * (u * u + v * v) + 0.5 * 9.81 * e * e) * dxdy;
-    mx = e > mx ? e : mx;
-    mn = e < mn ? e : mn;
-    mu = fabs(u) > mu ? fabs(u) : mu;
-    mv = fabs(v) > mv ? fabs(v) : mv;
-  }
+};
+template <class R>
+__device__ __forceinline__ void stats_cell(const Dev<R> &d, size_t o, double dxdy, StatAcc &a) {
+  if (d.MASK[o] < 1) return;
+  a.wet += 1.0;
+  double e = (double)d.Eta[o], u = (double)d.U[o], v = (double)d.V[o];
+  double h = e + (double)d.Depth[o];  // total depth formed in FP64
+  a.mass += e * dxdy;
+  a.en += (0.5 * h * (u * u + v * v) + 0.5 * 9.81 * e * e) * dxdy;
+  a.mx = e > a.mx ? e : a.mx;
+  a.mn = e < a.mn ? e : a.mn;
+  a.mu = fabs(u) > a.mu ? fabs(u) : a.mu;
+  a.mv = fabs(v) > a.mv ? fabs(v) : a.mv;
+}
+// workgroup reduction of the seven terms; thread 0 writes them to out[0..6]
+__device__ __forceinline__ void stats_block_reduce(const StatAcc &a, double *out) {
+  __shared__ double sm[7][256];
   int t = threadIdx.x;
-  sm[0][t] = mass; sm[1][t] = en; sm[2][t] = mx; sm[3][t] = mn; sm[4][t] = mu; sm[5][t] = mv; sm[6][t] = wet;
+  sm[0][t] = a.mass; sm[1][t] = a.en; sm[2][t] = a.mx; sm[3][t] = a.mn; sm[4][t] = a.mu; sm[5][t] = a.mv;
+  sm[6][t] = a.wet;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
     if (t < s) {
@@ -2291,7 +2412,44 @@ __global__ void k_stats(Dev<R> d, Geo g, Prm<R> p) {
     __syncthreads();
   }
   if (t == 0)
-    for (int k = 0; k < 7; k++) d.red64[blockIdx.x * 7 + k] = sm[k][0];
+    for (int k = 0; k < 7; k++) out[k] = sm[k][0];
+}
+template <class R>
+__global__ void k_stats(Dev<R> d, Geo g, Prm<R> p) {
+  StatAcc a;
+  int nx = g.Ie - g.Ib + 1, n = nx * (g.Je - g.Jb + 1);
+  double dxdy = (double)p.DX * (double)p.DY;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    int i = g.Ib + q % nx, j = g.Jb + q / nx;
+    stats_cell(d, IX(i, j), dxdy, a);
+  }
+  stats_block_reduce(a, d.red64 + blockIdx.x * 7);
+}
+
+// The same budget with one partial per interior row (workgroup j - Jbeg
+// reduces row j into part[7 * (j - Jbeg)]), then one workgroup sums the
+// rows in order into part[7 * rows .. 7 * rows + 6]: the partials live in
+// the caller's buffer (fw_reduce_stats_noalloc passes the library's static
+// device array).
+template <class R>
+__global__ void k_stats_rows(Dev<R> d, Geo g, Prm<R> p, double *part) {
+  StatAcc a;
+  const int j = g.Jb + blockIdx.x;
+  const double dxdy = (double)p.DX * (double)p.DY;
+  for (int i = g.Ib + threadIdx.x; i <= g.Ie; i += blockDim.x) stats_cell(d, IX(i, j), dxdy, a);
+  stats_block_reduce(a, part + 7 * blockIdx.x);
+}
+static __global__ void k_stats_sum(double *part, int rows) {
+  const int k = threadIdx.x;
+  if (k >= 7) return;
+  double s = k == 2 ? -1e300 : (k == 3 ? 1e300 : 0.0);
+  for (int r = 0; r < rows; r++) {
+    const double v = part[7 * r + k];
+    if (k == 2 || k == 4 || k == 5) s = v > s ? v : s;
+    else if (k == 3) s = v < s ? v : s;
+    else s += v;
+  }
+  part[7 * rows + k] = s;
 }
 
 #undef IX

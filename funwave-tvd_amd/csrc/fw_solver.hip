@@ -340,6 +340,15 @@ template <class R>
 int do_dispersion(fw_ctx *c) {
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
+  if (!c->prm<R>().full_state && !c->cfg.PERIODIC) {
+    // one pass: the second derivatives stay in LDS (k_disp_fused)
+    const dim3 b(kDispTX, kDispTY), gr((g.M + kDispTX - 1) / kDispTX, (g.N + kDispTY - 1) / kDispTY);
+    if (spacing_pow2<R>(c))
+      launch(c, "disp", k_disp_fused<R, true>, gr, b, d, g, c->prm<R>(), (R)c->dt);
+    else
+      launch(c, "disp", k_disp_fused<R, false>, gr, b, d, g, c->prm<R>(), (R)c->dt);
+    return FW_OK;
+  }
   if (spacing_pow2<R>(c))
     launch(c, "disp_deriv", k_disp_deriv<R, true>, grid2(g.M, g.N, kBT), kBT, d, g, c->prm<R>(), (R)c->dt);
   else

@@ -43,7 +43,11 @@ DIAGNOSTICS = {"P_center", "Q_center", "U_davg", "V_davg", "Wsurf", "ROLLER_FLUX
 # and the nonlinear dispersive terms that SourceTerms reads only as the sums
 # U1pp+U2+U3, V1pp+V2+V3 (kept as such)
 DISP_SUMMED = {"U1pp", "V1pp", "U2", "V2", "U3", "V3"}
-PRODUCTION_SKIP = FACE_STATES | DISP_FIRST | SOURCES | DIAGNOSTICS | DISP_SUMMED
+# and the second derivatives read only by CAL_DISPERSION's own terms (the
+# one-pass dispersion kernel keeps them on chip; the cross derivatives,
+# which ESTIMATE_HUV reads, are stored)
+DISP_SECOND = {"Uxx", "DUxx", "Vyy", "DVyy"}
+PRODUCTION_SKIP = FACE_STATES | DISP_FIRST | SOURCES | DIAGNOSTICS | DISP_SUMMED | DISP_SECOND
 
 
 def _compare(g, o, where, full_state=True):
