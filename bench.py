@@ -16,6 +16,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -69,6 +70,8 @@ def parse():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--host-math", action="store_true",
+                    help="Froude cap's ATAN2/COS/SIN by the host C library (FW_OPT_HOST_MATH)")
     ap.add_argument("--cpu-size", type=int, default=512)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--ensemble", action="store_true",
@@ -88,11 +91,14 @@ def parse():
 
 
 def ensemble(args):
-    """Learned coarse vs plain fine ensemble (conical-island INI_SOL members,
-    Mglob ~ U[256,768], Nglob ~ U[128,512], dx = 4 m; the learned runs use
-    dx = 8 m and the network correction after every step).  Prints one JSON
-    line; the learned rate counts only if every member's final eta is within
-    2 % relative L2 of the 2x2-averaged fine solution."""
+    """Learned coarse vs plain fine ensemble: conical-island solitary-wave
+    members (INI_SOL); the coarse grid (dx = 4 m, Mglob ~ U[256,768], Nglob
+    ~ U[128,512]) runs with the network correction after every step, the
+    fine grid (dx = 2 m, twice the cells each way) is the plain method it is
+    judged against and falls back to.  Each mode's members run together as
+    one ensemble (fw_batch).  Prints one JSON line; the learned rate counts
+    only if every member's delivered eta is within 2 % relative L2 of the
+    2x2-averaged fine solution."""
     import numpy as np
     import fwlearn
     t0 = time.time()
@@ -108,7 +114,7 @@ def ensemble(args):
     for seed in range(6000, 6000 + args.eval_members):
         c = fwlearn.member(seed)
         evals.append(fwlearn.with_fixed_dt(c, fwlearn.stable_dt(c)))
-    r = fwlearn.compare(evals, args.fine_steps, params, workers=args.workers)
+    r = fwlearn.compare_batched(evals, args.fine_steps, params)
     rph = r["runs_per_hour"]
     ok = r["all_learned_within_bound"]
     out = {
@@ -118,15 +124,18 @@ def ensemble(args):
         "accepted": ok, "plain_fine_runs_per_hour": round(rph["fine"], 1),
         "learned_runs_per_hour": round(rph["learned"], 1), "plain_coarse_runs_per_hour": round(rph["coarse"], 1),
         "speedup_learned_vs_fine": round(rph["learned"] / rph["fine"], 3),
-        "fallbacks_to_fine": r["fallbacks"],
+        "fallbacks_to_fine": r["fallbacks"], "members_improved": r["members_improved"],
         "max_err_learned": round(r["max_err_learned"], 4), "max_err_coarse": round(r["max_err_coarse"], 4),
-        "config": {"workload": "conical-island solitary-wave members (INI_SOL), fixed dt, "
-                               f"{args.fine_steps} fine steps each, {args.workers} members at a time "
-                               "(host threads, one context and stream pair per member); the correction "
-                               "applied without its wet-cell mean of eta (moves no water); a learned member "
-                               f"whose water volume drifts by more than {fwlearn.FALLBACK_DRIFT:g} or whose "
-                               f"energy grows by more than x{fwlearn.FALLBACK_ENERGY:g} falls back to its fine run "
-                               "(counted in the learned time)",
+        "wall_s": {k: round(v, 3) for k, v in r["wall_s"].items()},
+        "config": {"workload": "conical-island solitary-wave members (INI_SOL), fixed dt, coarse dx=4 m "
+                               "(Mglob~U[256,768], Nglob~U[128,512]) vs fine dx=2 m, "
+                               f"{args.fine_steps} fine steps ({args.fine_steps // 2} coarse) each; every mode's "
+                               "members advanced together (fw_batch, one launch per phase), the learned runs "
+                               "corrected after every step without the wet-cell mean of the eta correction "
+                               "(moves no water); a learned member whose water volume drifts by more than "
+                               f"{fwlearn.FALLBACK_DRIFT:g} or whose energy grows by more than "
+                               f"x{fwlearn.FALLBACK_ENERGY:g} falls back to its fine run (counted in the "
+                               "learned time)",
                    "train_members": args.train_members, "eval_members": args.eval_members,
                    "train_steps": args.train_steps, "loss_first": float(np.mean(losses[:20])),
                    "loss_last": float(np.mean(losses[-20:])),
@@ -330,9 +339,46 @@ def kernel_traffic_bytes(name, rb, M, N, mg, ng):
     return units * ((r + w) * rb + i * 4)
 
 
+def _reference_runs(ref, case, steps, warm, cores):
+    """Runs the compiled reference on `case` in `len(cores)` concurrent
+    processes, process k pinned to core cores[k], each with its own input
+    folder; returns [(steps, seconds)] from the reference's own timing
+    lines, or None if a run failed."""
+    import re
+    dirs, procs = [], []
+    try:
+        for c in cores:
+            td = tempfile.mkdtemp(prefix="fwref_")
+            dirs.append(td)
+            case.write(td)
+            env = dict(os.environ, FWREF_NSTEPS=str(warm + steps), FWREF_WARMUP=str(warm),
+                       MALLOC_PERTURB_="255", OMP_NUM_THREADS="1")
+            procs.append(subprocess.Popen(["bash", "-c", f"ulimit -s unlimited; exec taskset -c {c} {ref} input.txt"],
+                                          cwd=td, env=env, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                                          text=True))
+        out = []
+        for pr in procs:
+            so, _ = pr.communicate(timeout=900)
+            line = [ln for ln in so.splitlines() if ln.startswith("FWREF_TIMING")]
+            if pr.returncode != 0 or not line:
+                return None
+            mt = re.search(r"steps=\s*(\d+)\s+seconds=\s*([0-9.Ee+-]+)", line[0])
+            out.append((int(mt.group(1)), float(mt.group(2))))
+        return out
+    finally:
+        for pr in procs:
+            if pr.poll() is None:
+                pr.kill()
+        for td in dirs:
+            shutil.rmtree(td, ignore_errors=True)
+
+
 def cpu_baseline(args, precision):
     """The compiled reference (oracle/_ref) on a bounded sample of the same
-    workload family, timed with its own monotonic clock on one core."""
+    workload family, timed with its own monotonic clock: one process per
+    host core of this GPU's share (up to 16), each a member of its own, all
+    at once (the reference's serial build advances one domain per core), and
+    one process alone."""
     ref = os.path.join(REPO, "oracle", "_ref", f"fwref_{precision}")
     if not os.path.exists(ref):
         return None
@@ -340,26 +386,24 @@ def cpu_baseline(args, precision):
     n = args.cpu_size
     case = fwcases.beach_2d(mglob=n, nglob=n, dx=2.0)
     warm = 2
-    with tempfile.TemporaryDirectory() as td:
-        case.write(td)
-        env = dict(os.environ, FWREF_NSTEPS=str(warm + args.cpu_steps), FWREF_WARMUP=str(warm),
-                   MALLOC_PERTURB_="255", OMP_NUM_THREADS="1")
-        t0 = time.time()
-        r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec taskset -c 0 {ref} input.txt"],
-                           cwd=td, env=env, capture_output=True, text=True, timeout=600)
-        wall = time.time() - t0
-    line = [l for l in r.stdout.splitlines() if l.startswith("FWREF_TIMING")]
-    if r.returncode != 0 or not line:
-        return {"error": f"reference run failed rc={r.returncode}"}
-    import re
-    mt = re.search(r"steps=\s*(\d+)\s+seconds=\s*([0-9.Ee+-]+)", line[0])
-    steps, secs = int(mt.group(1)), float(mt.group(2))
-    return {"value": n * n * steps / secs / 1e6, "unit": "Mcell-steps/s", "cores": 1,
-            "kind": "reference",
-            "sample": f"reference FUNWAVE-TVD ({'FP64' if precision == 'dp' else 'FP32'} serial build, "
-                      f"amdflang -O2) on the same beach workload at {n}x{n} cells, dx=2 m: "
-                      f"{steps} steps after {warm} warm-up steps, {secs:.2f} s timed "
-                      f"({wall:.1f} s with set-up), one core"}
+    avail = sorted(os.sched_getaffinity(0))
+    cores = avail[:min(16, len(avail))]
+    t0 = time.time()
+    one = _reference_runs(ref, case, args.cpu_steps, warm, cores[:1])
+    many = _reference_runs(ref, case, args.cpu_steps, warm, cores) if len(cores) > 1 else one
+    wall = time.time() - t0
+    if not one or not many:
+        return {"error": "reference run failed"}
+    single = n * n * one[0][0] / one[0][1] / 1e6
+    agg = sum(n * n * st / sec / 1e6 for st, sec in many)
+    prec = "FP64" if precision == "dp" else "FP32"
+    return {"value": round(agg, 3), "unit": "Mcell-steps/s", "cores": len(cores), "kind": "reference",
+            "single_core": round(single, 3),
+            "sample": f"reference FUNWAVE-TVD ({prec} serial build, amdflang -O2) on the same beach workload at "
+                      f"{n}x{n} cells, dx=2 m, {args.cpu_steps} steps after {warm} warm-up steps: "
+                      f"{len(cores)} processes at once, one per core (cores {cores[0]}-{cores[-1]} of "
+                      f"{len(avail)} available), each advancing its own domain, summed; single_core: one "
+                      f"process alone ({wall:.1f} s wall for both runs with set-up)"}
 
 
 def main():
@@ -383,7 +427,7 @@ def main():
     n = args.size
     case = fwcases.beach_2d(mglob=n, nglob=n, dx=2.0)
     st = fwsetup.build_state(case, args.precision)
-    g = fwgpu.Solver(st, args.precision)
+    g = fwgpu.Solver(st, args.precision, host_math=args.host_math)
     t_setup = time.time() - t_setup
     rb = 8 if args.precision == "dp" else 4
     M, N = g.M, g.N
@@ -439,7 +483,9 @@ def main():
                                f"waves, {n}x{n} cells dx=2 m, sponges W/S/N, wetting-drying, "
                                f"dispersion + breaking, whole RK3 steps",
                    "Mglob": n, "Nglob": n, "precision": "FP64" if rb == 8 else "FP32",
-                   "line_solver": "thomas (bitwise the reference's recurrence)",
+                   "line_solver": "thomas, c' recurrence run ahead on a second stream (bitwise the "
+                                  "reference's recurrence)",
+                   "froude_cap_math": "host C library" if args.host_math else "device math library",
                    "parallelism": "one domain on one GPU"},
         # The whole step against its canonical algorithmic traffic:
         # 93 words per interior cell-step (372 B FP32, 744 B FP64), i.e.

@@ -961,6 +961,8 @@ int fw_batch_create(fw_ctx *const *members, int n, fw_batch **out) {
     if (!c->cfg.FIXED_DT || !(c->cfg.DT_fixed > 0))
       rc = bfail(b, FW_E_INVALID, "member " + std::to_string(m) + " has no FIXED_DT");
     if (full(c)) rc = bfail(b, FW_E_INVALID, "members run in production mode (FW_OPT_FULL_STATE = 0)");
+    if (c->pf.host_math || c->pd.host_math)
+      rc = bfail(b, FW_E_INVALID, "FW_OPT_HOST_MATH needs a host round trip per stage: not in a batch");
   }
   if (rc == FW_OK) {
     b->rb = c0->rb;

@@ -53,11 +53,11 @@ struct fw_ctx {
   int rb = 4;
   Geo g{};
   hipStream_t stream = nullptr;
-  // fw_step runs UPDATE_MASK and WAVE_BREAKING on a second stream while the
-  // line solves run (they share no array; see do_src_huv); cur is the stream
-  // launch() uses
-  hipStream_t side = nullptr, cur = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // fw_step runs UPDATE_MASK on a second stream and WAVE_BREAKING on a
+  // third while the line solves run (the three share no array; see
+  // do_src_huv); cur is the stream launch() uses
+  hipStream_t side = nullptr, side2 = nullptr, cur = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_brk = nullptr;
   int device = 0;
   std::vector<void *> allocs;
   std::vector<size_t> alloc_sizes;  // bytes of each allocation (snapshots)
@@ -126,6 +126,9 @@ template <> inline double hcos(double x) { return cos(x); }
 template <class R> void hsincos(R x, R &s, R &c);
 template <> inline void hsincos(float x, float &s, float &c) { sincosf(x, &s, &c); }
 template <> inline void hsincos(double x, double &s, double &c) { sincos(x, &s, &c); }
+template <class R> R hatan2(R y, R x);
+template <> inline float hatan2(float y, float x) { return atan2f(y, x); }
+template <> inline double hatan2(double y, double x) { return atan2(y, x); }
 template <class R> R htanh(R x);
 template <> inline float htanh(float x) { return tanhf(x); }
 template <> inline double htanh(double x) { return tanh(x); }

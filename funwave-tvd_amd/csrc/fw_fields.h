@@ -66,6 +66,13 @@ struct Dev {
   R *red;          // block partials
   double *red64;   // FP64 partials (stats)
   int *cand, *cand_n, *flags;
+  // per breaking candidate k: quadrant, px, py, pxy, ETAt, t2, t3 at
+  // cinf[8k..8k+6] (formed once by the first pass)
+  R *cinf;
+  // Froude-cap cells left to the host (host_math): count, cell indices,
+  // (u, v, Fr, max(H, MinDepthFrc)) per cell
+  int *fc_n, *fc_idx;
+  R *fc;
 };
 
 struct Geo {
@@ -82,6 +89,7 @@ struct Prm {
   int DIRECT_SPONGE, ETA_LIMITER, FIXED_DT, wavemaker, NumFreq;
   int ilo, ihi, jlo, jhi;
   int full_state;  // write every intermediate array (routine-level parity)
+  int host_math;   // Froude cap's ATAN2/COS/SIN evaluated on the host
   // 1/DX, 1/DY and whether they are exact (DX, DY powers of two): then
   // x/DX == x*(1/DX) bit for bit and the division is replaced
   R rDX, rDY;

@@ -15,9 +15,6 @@
 using namespace fw;
 using namespace fwi;
 
-
-
-
 namespace {
 
 hipEvent_t get_event(fw_ctx *c) {
@@ -154,8 +151,10 @@ int alloc_state(fw_ctx *c) {
   void **is[] = {(void **)&d.MASK, (void **)&d.MASK9, (void **)&d.MASK_STRUC, (void **)&d.MASKn, (void **)&d.cand};
   for (void **q : is)
     if ((rc = get(pe * sizeof(int), q)) != FW_OK) return rc;
+  if ((rc = get(8 * pe * sizeof(R), (void **)&d.cinf)) != FW_OK) return rc;
   if ((rc = get(64 * sizeof(int), (void **)&d.cand_n)) != FW_OK) return rc;
   d.flags = d.cand_n + 16;
+  d.fc_n = d.cand_n + 32;  // the Froude-cap list itself is allocated with FW_OPT_HOST_MATH
   if ((rc = get(kRedBlocks * sizeof(R), (void **)&d.red)) != FW_OK) return rc;
   if ((rc = get(kRedBlocks * 7 * sizeof(double), (void **)&d.red64)) != FW_OK) return rc;
   if ((rc = get(64 * sizeof(double), &c->scal)) != FW_OK) return rc;
@@ -444,7 +443,7 @@ int do_breaking(fw_ctx *c);
 template <class R>
 int do_roller(fw_ctx *c);
 
-// overlap: UPDATE_MASK and WAVE_BREAKING follow on the second stream,
+// overlap: UPDATE_MASK and WAVE_BREAKING follow on two more streams,
 // concurrently with the line solves
 template <class R>
 int do_src_huv(fw_ctx *c, int istage, bool overlap) {
@@ -463,24 +462,41 @@ int do_src_huv(fw_ctx *c, int istage, bool overlap) {
     launch(c, "src_huv", k_huv<R, true, false>, grid2(g.M, g.N, kBH), kBH, d, g, c->prm<R>(), alpha[istage - 1],
            beta[istage - 1], (R)c->dt, AA);
   if (!overlap) return do_solve<R>(c);
-  // UPDATE_MASK and WAVE_BREAKING read Eta, Depth, H, the dispersion slopes,
-  // the old MASK and AGE, and write MASKn, MASK9, DepthX/Y, AGEn, nu_break;
-  // the line solves and the velocity tail read the coefficients, H and the
-  // old MASK and write U, V, HU, HV, Ubar, Vbar.  Nothing is shared but the
-  // old MASK, which is double-buffered: the tail is launched first and keeps
-  // the old buffer; do_mask swaps the host pointers afterwards.
+  // UPDATE_MASK reads Eta, Depth, H and the old MASK and writes MASKn,
+  // MASK9, DepthX/Y; WAVE_BREAKING reads Eta, Depth, H, the dispersion
+  // slopes and AGE and writes AGEn, nu_break; the line solves and the
+  // velocity tail read the coefficients, H and the old MASK and write U, V,
+  // HU, HV, Ubar, Vbar.  Nothing is shared but the old MASK, which is
+  // double-buffered: the tail is launched first and keeps the old buffer;
+  // do_mask swaps the host pointers afterwards.  So the three run at once,
+  // UPDATE_MASK on the second stream, WAVE_BREAKING on the third.
   HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
   HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  HIPCHK(c, hipStreamWaitEvent(c->side2, c->ev_fork, 0));
   if ((rc = do_solve<R>(c))) return rc;
   c->cur = c->side;
   rc = do_mask<R>(c);
-  if (rc == FW_OK) rc = do_breaking<R>(c);
+  if (rc == FW_OK) {
+    hipError_t e = hipEventRecord(c->ev_join, c->side);
+    if (e != hipSuccess) rc = fail(c, FW_E_HIP, hipGetErrorString(e));
+  }
+  if (rc == FW_OK) {
+    c->cur = c->side2;
+    rc = do_breaking<R>(c);
+    if (rc == FW_OK) {
+      hipError_t e = hipEventRecord(c->ev_brk, c->side2);
+      if (e != hipSuccess) rc = fail(c, FW_E_HIP, hipGetErrorString(e));
+    }
+  }
   c->cur = c->stream;
   if (rc) return rc;
-  HIPCHK(c, hipEventRecord(c->ev_join, c->side));
   HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_brk, 0));
   return do_roller<R>(c);
 }
+
+template <class R>
+int froude_cap_on_host(fw_ctx *c);
 
 // line solves and the final velocities (GET_Eta_U_V_HU_HV after the
 // coefficient assembly)
@@ -501,10 +517,49 @@ int do_solve(fw_ctx *c) {
     }
   }
   launch(c, "uv_final", k_uv_final<R>, grid2(g.M, g.N, kB2), kB2, d, g, c->prm<R>());
+  if (c->prm<R>().host_math) {
+    int rc = froude_cap_on_host<R>(c);
+    if (rc) return rc;
+  }
   if (!c->cfg.DISPERSION) {
     size_t n = plane_elems(g);
     launch(c, "ubar_hu", k_ubar_from_hu<R>, dim3(1024), dim3(256), d, n);
   }
+  return FW_OK;
+}
+
+// The Froude cap (etauv_solver.F:580-586) of the cells k_uv_final listed,
+// evaluated with the host's C library as the reference does (ATAN2, COS,
+// SIN in the working precision), written back by k_fc_scatter.
+template <class R>
+int froude_cap_on_host(fw_ctx *c) {
+  Dev<R> &d = c->dev<R>();
+  int n = 0;
+  HIPCHK(c, hipMemcpyAsync(&n, d.fc_n, sizeof(int), hipMemcpyDeviceToHost, c->cur));
+  HIPCHK(c, hipStreamSynchronize(c->cur));
+  if (n == 0) return FW_OK;
+  std::vector<R> v(4 * (size_t)n);
+  HIPCHK(c, hipMemcpy(v.data(), d.fc, v.size() * sizeof(R), hipMemcpyDeviceToHost));
+  const R fcap = c->prm<R>().FCAP;
+  for (int k = 0; k < n; k++) {
+    const R u = v[4 * k], vv = v[4 * k + 1], fr = v[4 * k + 2], hm = v[4 * k + 3];
+    const R th = hatan2<R>(vv, u);
+    // COS(th) and SIN(th) as one sincos call, as the reference's compiler
+    // evaluates the pair (the FP64 results of separate calls can differ)
+    R sn, cs;
+    hsincos<R>(th, sn, cs);
+    const R un = fcap * fr * cs;
+    const R vn = fcap * fr * sn;
+    v[4 * k] = un;
+    v[4 * k + 1] = vn;
+    v[4 * k + 2] = un * hm;
+    v[4 * k + 3] = vn * hm;
+  }
+  HIPCHK(c, hipMemcpy(d.fc, v.data(), v.size() * sizeof(R), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_fc_scatter<R>, dim3((n + 255) / 256), dim3(256), 0, c->cur, d, (const R *)d.fc, n);
+  const int zero = 0;
+  HIPCHK(c, hipMemcpyAsync(d.fc_n, &zero, sizeof(int), hipMemcpyHostToDevice, c->cur));
+  HIPCHK(c, hipStreamSynchronize(c->cur));
   return FW_OK;
 }
 
@@ -659,9 +714,10 @@ int fw_create(const fw_config *cfg, fw_ctx **out) {
   c->g.Jb = f.Jbeg;
   c->g.Je = f.Jend;
   if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
-      hipStreamCreate(&c->side) != hipSuccess ||
+      hipStreamCreate(&c->side) != hipSuccess || hipStreamCreate(&c->side2) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_brk, hipEventDisableTiming) != hipSuccess) {
     fw_destroy(c);
     return FW_E_HIP;
   }
@@ -694,6 +750,7 @@ int fw_destroy(fw_ctx *c) {
   if (!c) return FW_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->side) (void)hipStreamSynchronize(c->side);
+  if (c->side2) (void)hipStreamSynchronize(c->side2);
   for (auto &p : c->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -705,7 +762,9 @@ int fw_destroy(fw_ctx *c) {
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_brk) (void)hipEventDestroy(c->ev_brk);
   if (c->side) (void)hipStreamDestroy(c->side);
+  if (c->side2) (void)hipStreamDestroy(c->side2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return FW_OK;
@@ -912,6 +971,29 @@ int fw_set_option(fw_ctx *c, int option, int value) {
   if (option == FW_OPT_FULL_STATE) {
     c->pf.full_state = value != 0;
     c->pd.full_state = value != 0;
+    return FW_OK;
+  }
+  if (option == FW_OPT_HOST_MATH) {
+    if (value && !c->dd.fc_idx) {
+      // list space for every interior cell, allocated on first use
+      const size_t cap = (size_t)(c->g.Ie - c->g.Ib + 1) * (size_t)(c->g.Je - c->g.Jb + 1);
+      void *idx = nullptr, *val = nullptr;
+      hipError_t e = hipMalloc(&idx, cap * sizeof(int));
+      if (e == hipSuccess) e = hipMalloc(&val, cap * 4 * (size_t)c->rb);
+      if (e != hipSuccess) {
+        if (idx) (void)hipFree(idx);
+        return fail(c, FW_E_NOMEM, "FW_OPT_HOST_MATH: list allocation failed");
+      }
+      c->allocs.push_back(idx);
+      c->alloc_sizes.push_back(cap * sizeof(int));
+      c->allocs.push_back(val);
+      c->alloc_sizes.push_back(cap * 4 * (size_t)c->rb);
+      c->dd.fc_idx = c->df.fc_idx = (int *)idx;
+      c->dd.fc = (double *)val;
+      c->df.fc = (float *)val;
+    }
+    c->pf.host_math = value != 0;
+    c->pd.host_math = value != 0;
     return FW_OK;
   }
   return fail(c, FW_E_INVALID, "unknown option");

@@ -28,6 +28,7 @@ DBL_CFG = ["DX", "DY", "DXg", "DYg", "CFL", "MinDepth", "MinDepthFrc", "Gamma1",
            "Time_ramp", "FreqPeak"]
 
 FW_OPT_FULL_STATE = 1
+FW_OPT_HOST_MATH = 2
 FW_OK, FW_E_INVALID, FW_E_HIP, FW_E_NOMEM, FW_E_BLOWUP, FW_E_NOTFOUND, FW_E_BREAK_ITER = 0, -1, -2, -3, -4, -5, -6
 WK = {"WK_REG": 1, "WK_DATA2D": 2}
 NO_SOURCE_WAVEMAKERS = ("INI_SOL", "INI_REC", "INI_GAU", "INI_OTH", "N_WAVE", "NONE")
@@ -146,7 +147,8 @@ def config_from_state(st: dict, real_bytes: int) -> Config:
 class Solver:
     """One HBM-resident FUNWAVE-TVD domain on the current GPU."""
 
-    def __init__(self, state: dict, precision: str = "sp", upload: bool = True, full_state: bool = False):
+    def __init__(self, state: dict, precision: str = "sp", upload: bool = True, full_state: bool = False,
+                 host_math: bool = False):
         self.lib = load_library()
         self.rb = 8 if precision == "dp" else 4
         self.rdt = np.float64 if self.rb == 8 else np.float32
@@ -158,6 +160,8 @@ class Solver:
         self.M, self.N = self.cfg.Mloc, self.cfg.Nloc
         if full_state:
             self._check(self.lib.fw_set_option(self.ctx, FW_OPT_FULL_STATE, 1))
+        if host_math:
+            self._check(self.lib.fw_set_option(self.ctx, FW_OPT_HOST_MATH, 1))
         if upload:
             self.upload_state(state)
 

@@ -141,6 +141,13 @@ int fw_reduce_stats(fw_ctx *ctx, fw_stats *out);
  * takes the nonlinear terms from fw_cal_dispersion's sums, not from
  * uploaded U1pp..V3). */
 enum { FW_OPT_FULL_STATE = 1 };
+/* FW_OPT_HOST_MATH = 1 evaluates the Froude cap's ATAN2, COS and SIN
+ * (etauv_solver.F:580-586) with the host's C library, as the reference does,
+ * instead of the device math library, which can differ in the last bit:
+ * the velocity tail lists the capped cells and the host rewrites them
+ * (one host round trip per line solve).  Runs are then bitwise the
+ * reference's over any number of steps; default 0. */
+enum { FW_OPT_HOST_MATH = 2 };
 int fw_set_option(fw_ctx *ctx, int option, int value);
 
 /* --- measurement -------------------------------------------------------- */

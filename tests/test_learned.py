@@ -7,7 +7,7 @@ import pytest
 
 def test_coarse_member_is_the_block_mean():
     import fwlearn
-    c = fwlearn.member(3, mglob=40, nglob=24, dt=0.1)
+    c = fwlearn.member(3, mglob=40, nglob=24, dt=0.1, r_base=20.0, h0=2.0)
     cc = fwlearn.coarse_case(c)
     assert (cc.mglob, cc.nglob) == (20, 12)
     assert cc.params["DX"] == 2 * c.params["DX"] and cc.params["DT_fixed"] == pytest.approx(0.2)
@@ -20,7 +20,7 @@ def test_coarse_member_is_the_block_mean():
 def test_fixed_step_reaches_the_host_state():
     import fwlearn
     import fwsetup
-    c = fwlearn.member(4, mglob=40, nglob=24)
+    c = fwlearn.member(4, mglob=40, nglob=24, r_base=20.0, h0=2.0)
     dt = fwlearn.stable_dt(c)
     st = fwsetup.build_state(fwlearn.with_fixed_dt(c, dt), "sp")
     assert st["FIXED_DT"] == 1 and st["DT_fixed"] == dt
@@ -29,19 +29,46 @@ def test_fixed_step_reaches_the_host_state():
 
 @pytest.mark.gpu
 def test_learned_workflow_end_to_end(gpu_available):
+    """Training data from four members, a network trained on them with the
+    project's own kernels, and two members the network has not seen: on
+    their one-step samples the network's correction must leave less error
+    than no correction, and in the batched comparison the corrected coarse
+    runs must end closer to the restricted fine solution than the plain
+    coarse runs, on average over the two (what the learned part gains over
+    the method without it), with finite, positive runs/hour."""
+    import fwcnn
     import fwlearn
-    print("\n[learned workflow] member 1, 96 x 64 fine cells", flush=True)
-    base = fwlearn.member(1, mglob=96, nglob=64)
-    case = fwlearn.with_fixed_dt(base, fwlearn.stable_dt(base))
-    x, y = fwlearn.make_samples(case, n_samples=6, every=2)
-    assert x.shape == (6, 5, 32, 48) and y.shape == (6, 3, 32, 48)
+    print("\n[learned workflow] members 1, 2, 4, 5 train; members 3, 6 held out; 96 x 64 fine cells", flush=True)
+
+    def case(seed):
+        base = fwlearn.member(seed, mglob=96, nglob=64, r_base=40.0)
+        return fwlearn.with_fixed_dt(base, fwlearn.stable_dt(base))
+
+    data = [fwlearn.make_samples(case(s), n_samples=12, every=2) for s in (1, 2, 4, 5)]
+    x, y = data[0]
+    assert x.shape == (12, 5, 32, 48) and y.shape == (12, 3, 32, 48)
     assert np.isfinite(x).all() and np.isfinite(y).all()
-    params, losses = fwlearn.train([(x, y)], steps=150, batch=8, size=32, lr=2e-3)
-    print(f"  loss {np.mean(losses[:10]):.3e} -> {np.mean(losses[-10:]):.3e}")
-    assert np.mean(losses[-10:]) < np.mean(losses[:10])
-    r = fwlearn.compare([case], fine_steps=40, params=params)
-    row = r["members"][0]
-    print(f"  errors vs restricted fine: coarse {row['err_coarse']:.3e}, learned {row['err_learned']:.3e}; "
-          f"runs/h {r['runs_per_hour']}")
-    assert np.isfinite(row["err_learned"]) and row["err_learned"] < 1.0
+    params, losses = fwlearn.train(data, steps=1500, batch=8, size=32, lr=2e-3)
+    print(f"  loss {np.mean(losses[:20]):.3e} -> {np.mean(losses[-20:]):.3e}")
+    assert np.mean(losses[-20:]) < np.mean(losses[:20])
+    held = [case(s) for s in (3, 6)]
+    # one-step: the held-out members' samples, corrected vs uncorrected
+    net = fwcnn.Net(32, 48, 12, params)
+    e_net, e_zero = 0.0, 0.0
+    for c in held:
+        hx, hy = fwlearn.make_samples(c, n_samples=12, every=2)
+        e_net += float(np.sum((net.forward(hx) - hy) ** 2))
+        e_zero += float(np.sum(hy ** 2))
+    net.close()
+    print(f"  held-out one-step squared error: corrected {e_net:.4e}, uncorrected {e_zero:.4e}")
+    assert e_net < e_zero
+    r = fwlearn.compare_batched(held, fine_steps=40, params=params)
+    for row in r["members"]:
+        print(f"  held-out {row['member']}: coarse error {row['err_coarse']:.3e}, "
+              f"learned {row['err_learned']:.3e}")
+    print(f"  runs/h {r['runs_per_hour']}")
+    errs_l = [row["err_learned"] for row in r["members"]]
+    errs_c = [row["err_coarse"] for row in r["members"]]
+    assert np.isfinite(errs_l).all()
+    assert np.mean(errs_l) < np.mean(errs_c)
     assert all(v > 0 for v in r["runs_per_hour"].values())
