@@ -41,6 +41,8 @@ KERNEL_TRAFFIC = {
     "disp_deriv": (6, 1, 11, "cell"),    # U V Depth Eta P Q, MASK9 -> 8 second derivatives, ETAT ETAx ETAy
     "disp_terms": (17, 1, 6, "cell"),    # 8 second derivatives, ETAT ETAx ETAy, U V U0 V0 Depth Eta
                                          #   -> U4 V4 U1p V1p and the sums U1pp+U2+U3, V1pp+V2+V3
+    "disp": (8, 1, 13, "cell"),          # one pass: U V U0 V0 Depth Eta P Q, MASK9 -> ETAT ETAx ETAy,
+                                         #   Uxy Vxy DUxy DVxy, U4 V4 U1p V1p and the two sums
     "sources": (24, 2, 2, "inner"),
     "huv": (24, 2, 12, "cell"),
     # SourceTerms fused into ESTIMATE_HUV: U V U4 V4 Cd H P Q Eta DepthX DepthY
@@ -286,7 +288,7 @@ def sync():
 # rocprofv3 names of the kernels behind each profiling name
 ROCPROF_NAME = {"trid": "k_trid_lds<", "src_huv": "k_huv<{r}, true,", "huv": "k_huv<{r}, false,",
                 "flux_x": "k_flux_x<", "flux_y": "k_flux_y", "disp_deriv": "k_disp_deriv<",
-                "disp_terms": "k_disp_terms<", "sources": "k_sources<", "uv_final": "k_uv_final<",
+                "disp_terms": "k_disp_terms<", "disp": "k_disp_fused<", "sources": "k_sources<", "uv_final": "k_uv_final<",
                 "mask": "k_mask<", "brk1": "k_brk1<", "mask_mul": "k_mask_mul<"}
 
 

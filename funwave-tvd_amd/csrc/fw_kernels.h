@@ -1637,13 +1637,9 @@ __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, 
         R t3 = gm * (p.cxa * dep * dep + p.cxb * dep * depr);
         R t4 = ddiv(ub * (R)mk, rmax(h, mdf)) + gm * (-(p.b1h * dep * dep * vxy) - b2 * dep * dvxy);
         if (t2 != (R)0 || mk > 0) {
-          d.xa[o] = ddiv(t1, t2);
-          d.xc[o] = ddiv(t3, t2);
-          d.xd[o] = ddiv(t4, t2);
+          d.xa[o] = ddiv(t1, t2); d.xc[o] = ddiv(t3, t2); d.xd[o] = ddiv(t4, t2);
         } else {
-          d.xa[o] = (R)0;
-          d.xc[o] = (R)0;
-          d.xd[o] = (R)0;
+          d.xa[o] = (R)0; d.xc[o] = (R)0; d.xd[o] = (R)0;
         }
       }
       {
@@ -1653,13 +1649,9 @@ __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, 
         R t3 = gm * (p.cya * dep * dep + p.cyb * dep * depr);
         R t4 = ddiv(vb * (R)mk, rmax(h, mdf)) + gm * (-(p.b1h * dep * dep * uxy) - b2 * dep * duxy);
         if (t2 != (R)0 || mk > 0) {
-          d.ya[o] = ddiv(t1, t2);
-          d.yc[o] = ddiv(t3, t2);
-          d.yd[o] = ddiv(t4, t2);
+          d.ya[o] = ddiv(t1, t2); d.yc[o] = ddiv(t3, t2); d.yd[o] = ddiv(t4, t2);
         } else {
-          d.ya[o] = (R)0;
-          d.yc[o] = (R)0;
-          d.yd[o] = (R)0;
+          d.ya[o] = (R)0; d.yc[o] = (R)0; d.yd[o] = (R)0;
         }
       }
     } else {
@@ -1773,19 +1765,13 @@ struct alignas(4 * sizeof(R)) TridRow {
 };
 template <class R>
 struct alignas(2 * sizeof(R)) TridCD {
-  R c, d;
-};
-template <class R>
-struct alignas(2 * sizeof(R)) TridDY {
-  R den, y;  // 1/a - c'_prev and RN(1/den) (FP32; 0 if not formed)
+  R c, d;  // during elimination d first holds the denominator 1/a - c'_prev
 };
 template <class R, int L, int CH>
 struct TridTile {
-  TridRow<R> v[3][L][CH + 1];   // forward input
-  TridCD<R> o[3][L][CH + 1];    // forward output (c', d') = backward input
-  TridDY<R> dy[3][L][CH + 1];   // the c' sweep's denominators for the d' sweep
-  R z[2][L][CH + 1];            // backward output
-  int exact[3];                 // the c' sweep of the chunk took the division
+  TridRow<R> v[3][L][CH + 1];  // forward input
+  TridCD<R> o[3][L][CH + 1];   // forward output = backward input
+  R z[2][L][CH + 1];           // backward output
 };
 
 // The serial lanes issue every instruction on their own (one wave per
@@ -1825,35 +1811,8 @@ constexpr int TG = 8;
 template <class R>
 struct TridDIn {
   TridRow<R> v;
-  TridDY<R> dy;
+  TridCD<R> o;
 };
-
-// Markstein's quotient x/den from y = RN(1/den) (FP32; see mdiv): q =
-// RN(x y), r = x - den q exactly, q' = RN(q + r y), with q's sign for x = 0.
-// Correctly rounded when den and x lie within 2^-62..2^62 or x = 0.
-__device__ __forceinline__ float mquot(float x, float den, float y) {
-  const float q = x * y;
-  const float q1 = __builtin_fmaf(__builtin_fmaf(-den, q, x), y, q);
-  return __builtin_copysignf(q1, q);
-}
-// RN(1/den) for den within 2^-62..2^62: v_rcp_f32 and one Newton step
-// (checked for every significand of those exponents, see mdiv)
-__device__ __forceinline__ float mrcp(float den) {
-  const float y0 = __builtin_amdgcn_rcpf(den);
-  return __builtin_fmaf(__builtin_fmaf(-den, y0, 1.0f), y0, y0);
-}
-// mrcp, or 0 for den outside 2^-62..2^62
-template <class R>
-__device__ __forceinline__ R rcp_or_zero(R den) {
-  const R a = dabs(den);
-  if (!(a >= (R)0x1p-62 && a <= (R)0x1p62)) return (R)0;
-  if constexpr (sizeof(R) == 4) return mrcp(den);
-  else return (R)1 / den;
-}
-__device__ __forceinline__ bool in_mrange(float x) {
-  const float a = __builtin_fabsf(x);
-  return a >= 0x1p-62f && a <= 0x1p62f;
-}
 
 template <class R, int L, int CH>
 __device__ __forceinline__ void trid_lds_body(Dev<R> d, Geo g, int nxblk, const Blk &B) {
@@ -1905,27 +1864,20 @@ __device__ __forceinline__ void trid_lds_body(Dev<R> d, Geo g, int nxblk, const 
       ra[q] = A[o]; rc[q] = CP[o]; rd[q] = DP[o];
     }
   };
-  // returns whether a row of the chunk that does not keep its C has c/a
-  // outside Markstein's range (nonzero and outside 2^-62..2^62).  Kept rows
-  // get r = 1: their denominator is formed and discarded, and 1 - c'_prev
-  // stays inside the range the c' sweep tests.
   auto fwd_put = [&](int ch, int b) {
-    bool out_of_range = false;
 #pragma unroll
     for (int q = 0; q < PER; q++) {
       if (ok(q, ch)) {
         const R a = ra[q];
         TridRow<R> v;
         if (ch * CH + sk[q] == 0 || !(a != (R)0)) {
-          v.r = (R)1; v.ca = rc[q]; v.da = rd[q]; v.keep = (R)1;
+          v.r = (R)0; v.ca = rc[q]; v.da = rd[q]; v.keep = (R)1;
         } else {
           v.r = ddiv((R)1, a); v.ca = ddiv(rc[q], a); v.da = ddiv(rd[q], a); v.keep = (R)0;
-          if constexpr (sizeof(R) == 4) out_of_range = out_of_range || !(v.ca == 0.0f || in_mrange(v.ca));
         }
         T.v[b][sl[q]][sk[q]] = v;
       }
     }
-    return out_of_range;
   };
   auto fwd_store = [&](int ch, int b) {
 #pragma unroll
@@ -1957,127 +1909,56 @@ __device__ __forceinline__ void trid_lds_body(Dev<R> d, Geo g, int nxblk, const 
   auto rows = [&](int ch) { return min(CH, n - ch * CH); };
 
   // ---- forward elimination, iteration it: wave 0 on chunk it, wave 1 on
-  // chunk it-1, staging as above.  The barrier after a chunk's put also
-  // tells every thread whether one of its c/a is outside Markstein's range.
-  bool put_oor = false;
+  // chunk it-1, staging as above
   if (w >= 2) {
     fwd_load(0);
-    put_oor = fwd_put(0, 0);
+    fwd_put(0, 0);
     if (nch > 1) fwd_load(1);
   }
-  bool ca_oor = __syncthreads_or(put_oor) != 0;
+  __syncthreads();
   R cprev = (R)0, dprev = (R)0;
   for (int it = 0; it <= nch; it++) {
-    put_oor = false;
     if (w == 0) {
       if (it < nch && lane < nl) {
         const TridRow<R> *row = T.v[it % 3][lane];
         TridCD<R> *out = T.o[it % 3][lane];
-        TridDY<R> *dyo = T.dy[it % 3][lane];
-        // the row-by-row recurrence with the division (tridiagnal.F:383-386)
-        auto exact = [&](int k, const TridRow<R> &v) {
+        auto step = [&](int k, const TridRow<R> &v) {
           const R den = v.r - cprev;
           R c = ddiv(v.ca, den);
           if (v.keep != (R)0) c = v.ca;
-          out[k].c = c;
-          dyo[k] = TridDY<R>{den, sizeof(R) == 4 ? rcp_or_zero(den) : (R)0};
+          out[k] = TridCD<R>{c, den};
           cprev = c;
         };
-        auto exact_group = [&](int k0, const TridRow<R>(&V)[TG]) {
+        auto group = [&](int k0, const TridRow<R>(&V)[TG]) {
 #pragma unroll
-          for (int q = 0; q < TG; q++) exact(k0 + q, V[q]);
+          for (int q = 0; q < TG; q++) step(k0 + q, V[q]);
         };
-        bool spec = false;
-        if constexpr (sizeof(R) == 4) spec = !ca_oor;
-        if (spec) {
-          // FP32: the chunk speculatively, c' = (c/a)/den by Markstein's
-          // quotient from y = RN(1/den) (kept for the d' sweep); the
-          // denominators' range is tracked beside the dependent chain and
-          // tested once after the chunk: if any line of the wave left
-          // 2^-62..2^62, the chunk again with the division from the same
-          // start (rows inside the range get the same values either way)
-          const R cstart = cprev;
-          float dmin = 0x1p62f, dmax = 0x1p-62f;
-          auto step = [&](int k, const TridRow<R> &v) {
-            const float den = v.r - cprev;
-            const float y = mrcp(den);
-            dmin = __builtin_fminf(dmin, __builtin_fabsf(den));
-            dmax = __builtin_fmaxf(dmax, __builtin_fabsf(den));
-            const float c = v.keep != (R)0 ? v.ca : mquot(v.ca, den, y);
-            out[k].c = c;
-            dyo[k] = TridDY<R>{den, y};
-            cprev = c;
-          };
-          auto group = [&](int k0, const TridRow<R>(&V)[TG]) {
-#pragma unroll
-            for (int q = 0; q < TG; q++) step(k0 + q, V[q]);
-          };
-          trid_rows<TG>(rows(it), [&](int k) { return row[k]; }, group, step);
-          spec = !__any(!(dmin >= 0x1p-62f && dmax <= 0x1p62f));
-          if (!spec) {
-            cprev = cstart;
-            trid_rows<TG>(rows(it), [&](int k) { return row[k]; }, exact_group, exact);
-          }
-        } else {
-          trid_rows<TG>(rows(it), [&](int k) { return row[k]; }, exact_group, exact);
-        }
-        if (lane == 0) T.exact[it % 3] = spec ? 0 : 1;
+        trid_rows<TG>(rows(it), [&](int k) { return row[k]; }, group, step);
       }
     } else if (w == 1) {
       if (it >= 1 && lane < nl) {
         const TridRow<R> *row = T.v[(it - 1) % 3][lane];
         TridCD<R> *io = T.o[(it - 1) % 3][lane];
-        const TridDY<R> *dyi = T.dy[(it - 1) % 3][lane];
-        auto ld = [&](int k) { return TridDIn<R>{row[k], dyi[k]}; };
-        auto exact = [&](int k, const TridDIn<R> &x) {
-          R dd = ddiv(x.v.da - dprev, x.dy.den);
+        auto step = [&](int k, const TridDIn<R> &x) {
+          R dd = ddiv(x.v.da - dprev, x.o.d);
           if (x.v.keep != (R)0) dd = x.v.da;
           io[k].d = dd;
           dprev = dd;
         };
-        auto exact_group = [&](int k0, const TridDIn<R>(&X)[TG]) {
+        auto group = [&](int k0, const TridDIn<R>(&X)[TG]) {
 #pragma unroll
-          for (int q = 0; q < TG; q++) exact(k0 + q, X[q]);
+          for (int q = 0; q < TG; q++) step(k0 + q, X[q]);
         };
-        bool spec = false;
-        if constexpr (sizeof(R) == 4) spec = T.exact[(it - 1) % 3] == 0;
-        if (spec) {
-          // FP32, the chunk's denominators all inside the range (the c'
-          // sweep took Markstein's quotients): d' = (d/a - d'_prev)/den
-          // from the same y, speculatively; the dividends' range (largest
-          // |x|, smallest nonzero |x|) is tested once after the chunk
-          const R dstart = dprev;
-          float amax = 0.0f, amin = 0x1p62f;
-          auto step = [&](int k, const TridDIn<R> &x) {
-            const float xx = x.v.da - dprev;
-            const float ax = __builtin_fabsf(xx);
-            amax = __builtin_fmaxf(amax, ax);
-            amin = __builtin_fminf(amin, xx == 0.0f ? 0x1p62f : ax);
-            const float dd = x.v.keep != (R)0 ? x.v.da : mquot(xx, x.dy.den, x.dy.y);
-            io[k].d = dd;
-            dprev = dd;
-          };
-          auto group = [&](int k0, const TridDIn<R>(&X)[TG]) {
-#pragma unroll
-            for (int q = 0; q < TG; q++) step(k0 + q, X[q]);
-          };
-          trid_rows<TG>(rows(it - 1), ld, group, step);
-          if (__any(!(amax <= 0x1p62f && amin >= 0x1p-62f))) {
-            dprev = dstart;
-            trid_rows<TG>(rows(it - 1), ld, exact_group, exact);
-          }
-        } else {
-          trid_rows<TG>(rows(it - 1), ld, exact_group, exact);
-        }
+        trid_rows<TG>(rows(it - 1), [&](int k) { return TridDIn<R>{row[k], io[k]}; }, group, step);
       }
     } else {
       if (it + 1 < nch) {
-        put_oor = fwd_put(it + 1, (it + 1) % 3);
+        fwd_put(it + 1, (it + 1) % 3);
         if (it + 2 < nch) fwd_load(it + 2);
       }
       if (it >= 2 && it - 2 < nch) fwd_store(it - 2, (it - 2) % 3);
     }
-    ca_oor = __syncthreads_or(put_oor) != 0;
+    __syncthreads();
   }
   // ---- back substitution, chunks in reverse; chunks nch-1..nch-3 are in
   // LDS, the staging waves put chunk ch-1 (loaded during the previous chunk)

@@ -29,29 +29,32 @@ def test_fixed_step_reaches_the_host_state():
 
 @pytest.mark.gpu
 def test_learned_workflow_end_to_end(gpu_available):
-    """Training data from four members, a network trained on them with the
-    project's own kernels, and two members the network has not seen: on
-    their one-step samples the network's correction must leave less error
-    than no correction, and in the batched comparison the corrected coarse
-    runs must end closer to the restricted fine solution than the plain
-    coarse runs, on average over the two (what the learned part gains over
-    the method without it), with finite, positive runs/hour."""
+    """Training data from sixteen members (192 x 128 fine cells), a network
+    trained on them with the project's own kernels, and four members of
+    another size (96 x 64) that the network has not seen: on their one-step
+    samples the corrected coarse step must leave less error than the plain
+    coarse step, and in the batched comparison over 40 fine steps the
+    corrected coarse runs must end closer to the restricted fine solution
+    than the plain coarse runs, on average over the four (what the learned
+    part gains over the method without it), with finite, positive
+    runs/hour."""
     import fwcnn
     import fwlearn
-    print("\n[learned workflow] members 1, 2, 4, 5 train; members 3, 6 held out; 96 x 64 fine cells", flush=True)
+    print("\n[learned workflow] members 20..35 (192 x 128) train; members 3, 6, 13, 14 (96 x 64) held out",
+          flush=True)
 
-    def case(seed):
-        base = fwlearn.member(seed, mglob=96, nglob=64, r_base=40.0)
+    def case(seed, m, n, r):
+        base = fwlearn.member(seed, mglob=m, nglob=n, r_base=r)
         return fwlearn.with_fixed_dt(base, fwlearn.stable_dt(base))
 
-    data = [fwlearn.make_samples(case(s), n_samples=12, every=2) for s in (1, 2, 4, 5)]
+    data = [fwlearn.make_samples(case(s, 192, 128, 80.0), n_samples=16, every=2) for s in range(20, 36)]
     x, y = data[0]
-    assert x.shape == (12, 5, 32, 48) and y.shape == (12, 3, 32, 48)
+    assert x.shape == (16, 5, 64, 96) and y.shape == (16, 3, 64, 96)
     assert np.isfinite(x).all() and np.isfinite(y).all()
-    params, losses = fwlearn.train(data, steps=1500, batch=8, size=32, lr=2e-3)
+    params, losses = fwlearn.train(data, steps=6000, batch=16, size=32, lr=2e-3)
     print(f"  loss {np.mean(losses[:20]):.3e} -> {np.mean(losses[-20:]):.3e}")
     assert np.mean(losses[-20:]) < np.mean(losses[:20])
-    held = [case(s) for s in (3, 6)]
+    held = [case(s, 96, 64, 40.0) for s in (3, 6, 13, 14)]
     # one-step: the held-out members' samples, corrected vs uncorrected
     net = fwcnn.Net(32, 48, 12, params)
     e_net, e_zero = 0.0, 0.0
