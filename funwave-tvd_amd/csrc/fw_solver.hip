@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -14,6 +15,40 @@
 
 using namespace fw;
 using namespace fwi;
+
+// ---- the allocation-free budget (fw_reduce_stats_noalloc) ----------------
+// Partials of the per-row reduction live in this file-scope device array:
+// one row of seven doubles per interior row, up to kNoallocRows rows, and
+// the seven totals after them (458 KB, held on every GPU the library runs
+// on until the process ends).  Host code never passes the array's name to a
+// kernel (in host code it is a stand-in, not a device address): fw_create
+// resolves its address on each device once (hipGetSymbolAddress, with that
+// device current) and keeps it in g_scratch_ptr[device].
+constexpr int kNoallocRows = FW_NOALLOC_MAX_ROWS;
+static __device__ __align__(256) double scratch[7 * kNoallocRows + 8];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of scratch on device dev (current): resolved once
+int resolve_scratch(int dev, std::string &err) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    err = "device index beyond the static scratch table";
+    return FW_E_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return FW_OK;
+  double *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return FW_E_HIP;
+  }
+  g_scratch_ptr[dev] = ptr;
+  return FW_OK;
+}
+}  // namespace
 
 namespace {
 
@@ -659,6 +694,40 @@ int do_stats(fw_ctx *c, fw_stats *out) {
   return FW_OK;
 }
 
+// fw_reduce_stats without the context's partials buffer: one partial per
+// interior row in the static scratch array, summed on the device, seven
+// totals copied to the host (no allocation of any kind on this path)
+template <class R>
+int do_stats_noalloc(fw_ctx *c, fw_stats *out) {
+  const Geo &g = c->g;
+  const int rows = g.Je - g.Jb + 1;
+  if (rows > kNoallocRows) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "fw_reduce_stats_noalloc: %d interior rows exceed the static scratch of %d rows",
+             rows, kNoallocRows);
+    return fail(c, FW_E_INVALID, buf);
+  }
+  double *part = g_scratch_ptr[c->device];
+  if (!part) return fail(c, FW_E_INVALID, "static scratch not resolved on this device");
+  // plain launches: the profiling wrapper would record events
+  hipLaunchKernelGGL(k_stats_rows<R>, dim3(rows), dim3(256), 0, c->stream, c->dev<R>(), g, c->prm<R>(), part);
+  hipLaunchKernelGGL(k_stats_sum, dim3(1), dim3(64), 0, c->stream, part, rows);
+  HIPCHK(c, hipGetLastError());
+  double h[7];
+  HIPCHK(c, hipMemcpyAsync(h, part + 7 * rows, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  fw_stats st{};
+  st.mass_volume = h[0];
+  st.energy = h[1];
+  st.max_eta = h[2];
+  st.min_eta = h[3];
+  st.max_u = h[4];
+  st.max_v = h[5];
+  st.wet_cells = (int64_t)h[6];
+  *out = st;
+  return FW_OK;
+}
+
 // dispatch on the working precision
 #define DISPATCH(c, call) ((c)->rb == 8 ? call<double> : call<float>)
 
@@ -722,7 +791,11 @@ int fw_create(const fw_config *cfg, fw_ctx **out) {
     return FW_E_HIP;
   }
   c->cur = c->stream;
-  int rc;
+  int rc = resolve_scratch(c->device, c->err);
+  if (rc != FW_OK) {
+    fw_destroy(c);
+    return rc;
+  }
   if (c->rb == 8) {
     fill_prm<double>(c);
     rc = alloc_state<double>(c);
@@ -964,6 +1037,11 @@ int fw_step(fw_ctx *c, int nsteps, double *dt, double *time) {
 int fw_reduce_stats(fw_ctx *c, fw_stats *out) {
   if (!c || !out) return FW_E_INVALID;
   return DISPATCH(c, do_stats)(c, out);
+}
+
+int fw_reduce_stats_noalloc(fw_ctx *c, fw_stats *out) {
+  if (!c || !out) return FW_E_INVALID;
+  return DISPATCH(c, do_stats_noalloc)(c, out);
 }
 
 int fw_set_option(fw_ctx *c, int option, int value) {

@@ -44,7 +44,7 @@ EXPORTS = ["fw_create", "fw_destroy", "fw_strerror", "fw_synchronize",
            "fw_profile_names", "fw_time_steps", "fw_field_device", "fw_get_stream",
            "fw_get_config", "fw_batch_create", "fw_batch_destroy", "fw_batch_strerror",
            "fw_batch_size", "fw_batch_step", "fw_batch_status", "fw_batch_time_steps",
-           "fw_batch_set_graph", "fw_batch_autotune", "fw_batch_get_shapes"]
+           "fw_batch_set_graph", "fw_batch_autotune", "fw_batch_get_shapes", "fw_reduce_stats_noalloc"]
 FW_BATCH_RUNNING, FW_BATCH_BLOWUP, FW_BATCH_DT_CHANGED, FW_BATCH_BREAK_ITER = 0, 1, 2, 3
 
 
@@ -98,6 +98,7 @@ def load_library(path: str = LIB_PATH):
     lib.fw_check_blowup.argtypes = [vp, dp]
     lib.fw_step.argtypes = [vp, ip, dp, dp]
     lib.fw_reduce_stats.argtypes = [vp, ct.POINTER(Stats)]
+    lib.fw_reduce_stats_noalloc.argtypes = [vp, ct.POINTER(Stats)]
     lib.fw_profile_enable.argtypes = [vp, ip]
     lib.fw_set_option.argtypes = [vp, ip, ip]
     lib.fw_profile_get.argtypes = [vp, ct.c_char_p, ct.POINTER(ct.c_int64), dp]
@@ -287,6 +288,12 @@ class Solver:
     def stats(self):
         s = Stats()
         self._check(self.lib.fw_reduce_stats(self.ctx, ct.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def stats_noalloc(self):
+        """fw_reduce_stats_noalloc: the same budget with no allocation"""
+        s = Stats()
+        self._check(self.lib.fw_reduce_stats_noalloc(self.ctx, ct.byref(s)))
         return {k: getattr(s, k) for k, _ in Stats._fields_}
 
     def synchronize(self):

@@ -123,6 +123,19 @@ int fw_check_blowup(fw_ctx *ctx, double *max_abs_eta);
 int fw_step(fw_ctx *ctx, int nsteps, double *dt, double *time);
 /* FP64 mass / energy / extrema budget of the interior. */
 int fw_reduce_stats(fw_ctx *ctx, fw_stats *out);
+/* The same budget for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): no
+ * counterpart in the reference.  It allocates nothing, device or host: the
+ * per-row partials go to a static device array of the library, sized for
+ * at most FW_NOALLOC_MAX_ROWS interior rows (Nloc - 6); taller domains are
+ * refused with FW_E_INVALID.  Every call on one GPU shares that array, so
+ * calls that overlap on one GPU (from several threads or contexts) race;
+ * each call writes every partial it reads, so nothing carries over from the
+ * last call.  Sums are taken in another order than fw_reduce_stats', so the
+ * FP64 mass and energy can differ in the last bits; extrema and the wet-cell
+ * count are identical. */
+#define FW_NOALLOC_MAX_ROWS 8192
+int fw_reduce_stats_noalloc(fw_ctx *ctx, fw_stats *out);
 
 /* --- options ------------------------------------------------------------ */
 /* FW_OPT_FULL_STATE = 1 keeps every intermediate array of MODULE GLOBAL up to
