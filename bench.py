@@ -78,9 +78,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--ensemble", action="store_true",
                     help="measure learned coarse vs plain fine ensemble runs/hour instead")
-    ap.add_argument("--train-members", type=int, default=8)
+    ap.add_argument("--train-members", type=int, default=16)
     ap.add_argument("--eval-members", type=int, default=8)
-    ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--fine-steps", type=int, default=200)
     ap.add_argument("--workers", type=int, default=8,
                     help="ensemble members run at a time (host threads)")
