@@ -90,6 +90,8 @@ struct Prm {
   int ilo, ihi, jlo, jhi;
   int full_state;  // write every intermediate array (routine-level parity)
   int host_math;   // Froude cap's ATAN2/COS/SIN evaluated on the host
+  int xd_in_huv;   // the RK update forms the cross derivatives itself (the
+                   // one-pass dispersion kernel does not store them)
   // 1/DX, 1/DY and whether they are exact (DX, DY powers of two): then
   // x/DX == x*(1/DX) bit for bit and the division is replaced
   R rDX, rDY;

@@ -735,7 +735,7 @@ __device__ __forceinline__ void disp_fused_body(Dev<R> d, Geo g, Prm<R> p, R DT,
   const R uxy = sS[2][ly][lx], vxy = sS[1][ly][lx], duxy = sS[6][ly][lx], dvxy = sS[5][ly][lx];
   disp_terms_at<R, P2>(d, g, p, DT, i, j, S);
   if (has_etat) d.ETAT[o] = etat;
-  d.Uxy[o] = uxy; d.Vxy[o] = vxy; d.DUxy[o] = duxy; d.DVxy[o] = dvxy;
+  if (!p.xd_in_huv) { d.Uxy[o] = uxy; d.Vxy[o] = vxy; d.DUxy[o] = duxy; d.DVxy[o] = dvxy; }
   if (inner ? (g2 || p.SHOW_BRK) : g2) {
     d.ETAx[o] = etax;
     d.ETAy[o] = etay;
@@ -1558,7 +1558,13 @@ __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
 // FUSED: SourceTerms of the cell are formed here (source_terms) instead of
 // read back, which is the same arithmetic on the same inputs: SourceTerms
 // reads nothing ESTIMATE_HUV writes.
-template <class R, bool FUSED, bool P2>
+// XD: the four cross derivatives Uxy, Vxy, DUxy, DVxy, which only the line
+// coefficients of this cell read, are formed here from U, V, Depth and MASK9
+// (unchanged since CAL_DISPERSION) instead of read from the planes the
+// dispersion kernel would otherwise write: the same operations on the same
+// operands as there (an interior cell is its own mirror cell; wall cells
+// take 0).
+template <class R, bool FUSED, bool P2, bool XD = false>
 __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA, const Blk &B) {
   const Tile tb = xcd_tile(B);
   int i = 1 + tb.x * blockDim.x + threadIdx.x;
@@ -1587,8 +1593,25 @@ __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, 
     const R m9 = (R)(disp ? d.MASK9 : d.MASK)[o];
     const R dxl = d.Depth[disp ? IX(i - 1, j) : o], dxr = d.Depth[disp ? IX(i + 1, j) : o];
     const R dyl = d.Depth[disp ? IX(i, j - 1) : o], dyr = d.Depth[disp ? IX(i, j + 1) : o];
-    const R vxy = (disp ? d.Vxy : d.Depth)[o], dvxy = (disp ? d.DVxy : d.Depth)[o];
-    const R uxy = (disp ? d.Uxy : d.Depth)[o], duxy = (disp ? d.DUxy : d.Depth)[o];
+    R vxy, dvxy, uxy, duxy;
+    if constexpr (XD) {
+      const R *U = d.U, *V = d.V, *Dp = d.Depth;
+      auto DU = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * U[q]; };
+      auto DV = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * V[q]; };
+      const Dv<R> dx = dvx<P2>(p), dy = dvy<P2>(p);
+      const bool wall = (i == g.Ib || i == g.Ie || j == g.Jb || j == g.Je);
+      const R uxy_s = dfxy(g, U, i, j, dx, dy) * m9, vxy_s = dfxy(g, V, i, j, dx, dy) * m9;
+      const R duxy_s = gfxy<R>(DU, i, j, dx, dy) * m9, dvxy_s = gfxy<R>(DV, i, j, dx, dy) * m9;
+      uxy = wall ? (R)0 : uxy_s;
+      vxy = wall ? (R)0 : vxy_s;
+      duxy = wall ? (R)0 : duxy_s;
+      dvxy = wall ? (R)0 : dvxy_s;
+    } else {
+      vxy = (disp ? d.Vxy : d.Depth)[o];
+      dvxy = (disp ? d.DVxy : d.Depth)[o];
+      uxy = (disp ? d.Uxy : d.Depth)[o];
+      duxy = (disp ? d.DUxy : d.Depth)[o];
+    }
     R srcx, srcy, wmm;
     if (FUSED) {
       source_terms<R, P2>(d, g, p, AA, i, j, p.full_state != 0, p.full_state == 0, srcx, srcy, wmm);
@@ -1662,9 +1685,9 @@ __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, 
     d.H[o] = eta * p.G3 + d.Depth[o];
   }
 }
-template <class R, bool FUSED, bool P2>
+template <class R, bool FUSED, bool P2, bool XD = false>
 __global__ void __launch_bounds__(WideLB<R>::wg256) k_huv(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA) {
-  huv_body<R, FUSED, P2>(d, g, p, al, be, DT, AA, grid_blk());
+  huv_body<R, FUSED, P2, XD>(d, g, p, al, be, DT, AA, grid_blk());
 }
 
 // TRIDx_ser / TRIDy_ser (tridiagnal.F:338-398): Thomas sweeps on rows

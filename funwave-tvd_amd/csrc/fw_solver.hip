@@ -370,17 +370,28 @@ int do_max_eta(fw_ctx *c, double *out) {
   return FW_OK;
 }
 
+// whether the one-pass dispersion kernel runs (production, non-periodic);
+// in fw_step the RK update then forms the cross derivatives itself
 template <class R>
-int do_dispersion(fw_ctx *c) {
+bool disp_one_pass(fw_ctx *c) {
+  return !c->prm<R>().full_state && !c->cfg.PERIODIC;
+}
+
+// in_step: called by fw_step, whose RK update forms the cross derivatives
+// (the per-routine ESTIMATE_HUV reads them from the arrays)
+template <class R>
+int do_dispersion(fw_ctx *c, bool in_step = false) {
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
-  if (!c->prm<R>().full_state && !c->cfg.PERIODIC) {
+  if (disp_one_pass<R>(c)) {
     // one pass: the second derivatives stay in LDS (k_disp_fused)
     const dim3 b(kDispTX, kDispTY), gr((g.M + kDispTX - 1) / kDispTX, (g.N + kDispTY - 1) / kDispTY);
+    Prm<R> p = c->prm<R>();
+    p.xd_in_huv = in_step ? 1 : 0;
     if (spacing_pow2<R>(c))
-      launch(c, "disp", k_disp_fused<R, true>, gr, b, d, g, c->prm<R>(), (R)c->dt);
+      launch(c, "disp", k_disp_fused<R, true>, gr, b, d, g, p, (R)c->dt);
     else
-      launch(c, "disp", k_disp_fused<R, false>, gr, b, d, g, c->prm<R>(), (R)c->dt);
+      launch(c, "disp", k_disp_fused<R, false>, gr, b, d, g, p, (R)c->dt);
     return FW_OK;
   }
   if (spacing_pow2<R>(c))
@@ -490,12 +501,24 @@ int do_src_huv(fw_ctx *c, int istage, bool overlap) {
   if (rc) return rc;
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
-  if (spacing_pow2<R>(c))
-    launch(c, "src_huv", k_huv<R, true, true>, grid2(g.M, g.N, kBH), kBH, d, g, c->prm<R>(), alpha[istage - 1],
-           beta[istage - 1], (R)c->dt, AA);
-  else
-    launch(c, "src_huv", k_huv<R, true, false>, grid2(g.M, g.N, kBH), kBH, d, g, c->prm<R>(), alpha[istage - 1],
-           beta[istage - 1], (R)c->dt, AA);
+  // with the one-pass dispersion of fw_step the update forms the cross
+  // derivatives itself
+  const bool xd = c->cfg.DISPERSION && disp_one_pass<R>(c);
+  if (spacing_pow2<R>(c)) {
+    if (xd)
+      launch(c, "src_huv", k_huv<R, true, true, true>, grid2(g.M, g.N, kBH), kBH, d, g, c->prm<R>(),
+             alpha[istage - 1], beta[istage - 1], (R)c->dt, AA);
+    else
+      launch(c, "src_huv", k_huv<R, true, true>, grid2(g.M, g.N, kBH), kBH, d, g, c->prm<R>(), alpha[istage - 1],
+             beta[istage - 1], (R)c->dt, AA);
+  } else {
+    if (xd)
+      launch(c, "src_huv", k_huv<R, true, false, true>, grid2(g.M, g.N, kBH), kBH, d, g, c->prm<R>(),
+             alpha[istage - 1], beta[istage - 1], (R)c->dt, AA);
+    else
+      launch(c, "src_huv", k_huv<R, true, false>, grid2(g.M, g.N, kBH), kBH, d, g, c->prm<R>(), alpha[istage - 1],
+             beta[istage - 1], (R)c->dt, AA);
+  }
   if (!overlap) return do_solve<R>(c);
   // UPDATE_MASK reads Eta, Depth, H and the old MASK and writes MASKn,
   // MASK9, DepthX/Y; WAVE_BREAKING reads Eta, Depth, H, the dispersion
@@ -652,7 +675,7 @@ int do_step(fw_ctx *c, int nsteps) {
     if ((rc = do_exchange<R>(c, false))) return rc;
     if ((rc = do_estimate_dt<R>(c, true))) return rc;
     for (int st = 1; st <= 3; st++) {
-      if (f.DISPERSION && (rc = do_dispersion<R>(c))) return rc;
+      if (f.DISPERSION && (rc = do_dispersion<R>(c, true))) return rc;
       if ((rc = do_fluxes<R>(c))) return rc;
       if ((rc = do_src_huv<R>(c, st, true))) return rc;
       if ((rc = do_exchange<R>(c, f.DIRECT_SPONGE != 0))) return rc;
@@ -1004,7 +1027,7 @@ int fw_save_stage0(fw_ctx *c, int which) {
   }
   return FW_OK;
 }
-int fw_cal_dispersion(fw_ctx *c) { ENTRY(c, DISPATCH(c, do_dispersion)(c)); }
+int fw_cal_dispersion(fw_ctx *c) { ENTRY(c, DISPATCH(c, do_dispersion)(c, false)); }
 int fw_fluxes(fw_ctx *c) { ENTRY(c, DISPATCH(c, do_fluxes)(c)); }
 int fw_source_terms(fw_ctx *c) { ENTRY(c, DISPATCH(c, do_sources)(c)); }
 int fw_estimate_huv(fw_ctx *c, int istage) { ENTRY(c, DISPATCH(c, do_huv)(c, istage)); }

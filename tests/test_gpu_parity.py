@@ -48,13 +48,19 @@ DISP_SUMMED = {"U1pp", "V1pp", "U2", "V2", "U3", "V3"}
 # which ESTIMATE_HUV reads, are stored)
 DISP_SECOND = {"Uxx", "DUxx", "Vyy", "DVyy"}
 PRODUCTION_SKIP = FACE_STATES | DISP_FIRST | SOURCES | DIAGNOSTICS | DISP_SUMMED | DISP_SECOND
+# and, in fw_step's production path, the cross derivatives, which the RK
+# update forms itself at its own cell (the one-pass dispersion does not
+# store them; the per-routine CAL_DISPERSION does)
+DISP_CROSS = {"Uxy", "Vxy", "DUxy", "DVxy"}
 
 
-def _compare(g, o, where, full_state=True):
+def _compare(g, o, where, full_state=True, step=False):
     import fwgpu
     bad = {}
     for k in fwgpu.ALL_FIELDS:
         if k in fwgpu.AUX or (not full_state and k in PRODUCTION_SKIP):
+            continue
+        if step and not full_state and k in DISP_CROSS:
             continue
         n = bitwise_mismatch(g.download(k), o[k])
         if n:
@@ -162,7 +168,7 @@ def test_fused_step_matches_oracle(name, prec, full_state, gpu_available):
         gdt, gt = g.step(n)
         o.run(n)
         assert (gdt, gt) == (o.dt.value, o.time.value)
-        _compare(g, o, f"after fw_step({n})", full_state)
+        _compare(g, o, f"after fw_step({n})", full_state, step=True)
     g.close()
 
 
@@ -252,7 +258,7 @@ def test_full_size_beach_step_matches_oracle(prec, gpu_available):
     o = oracle.OracleSolver(st)
     o.run(1)
     assert (gdt, gt) == (o.dt.value, o.time.value)
-    _compare(g, o, "after fw_step(1) at 2048^2", full_state=False)
+    _compare(g, o, "after fw_step(1) at 2048^2", full_state=False, step=True)
     g.close()
 
 
