@@ -188,8 +188,10 @@ __global__ void __launch_bounds__(64 * kFluxYThreadRows) kb_flux_y(FWB_ARGS, int
 }
 
 // SourceTerms + ESTIMATE_HUV of one stage; the wavemaker factors of the
-// member's current step come from its table
-template <class R, bool P2>
+// member's current step come from its table; XD: as in fw_step, the update
+// forms the cross derivatives and U1p, V1p itself (the one-pass dispersion
+// does not store them)
+template <class R, bool P2, bool XD = false>
 __global__ void __launch_bounds__(WideLB<R>::wg256) kb_huv(FWB_ARGS, int pm, int pa, R al, R be) {
   FWB_HEAD
   Dev<R> d = mdev(M, pm, pa);
@@ -199,7 +201,7 @@ __global__ void __launch_bounds__(WideLB<R>::wg256) kb_huv(FWB_ARGS, int pm, int
     AA = row[0];
     d.wk_stage = const_cast<R *>(row + 1);
   }
-  huv_body<R, true, P2>(d, M.g, M.p, al, be, M.clk[0], AA, tile_blk(e));
+  huv_body<R, true, P2, XD>(d, M.g, M.p, al, be, M.clk[0], AA, tile_blk(e));
 }
 
 template <class R, int L, int CH>
@@ -460,6 +462,9 @@ int fill_members(fw_batch *b) {
     M.d = c->dev<R>();
     M.g = c->g;
     M.p = c->prm<R>();
+    // the one-pass dispersion leaves the cross derivatives and U1p, V1p to
+    // the RK update (as in fw_step)
+    M.p.xd_in_huv = (c->cfg.DISPERSION && !c->cfg.PERIODIC) ? 1 : 0;
     M.show_brk = c->cfg.SHOW_BREAKING;
     Dev<R> d1;
     swapped(M.d, d1, M.show_brk != 0);
@@ -596,7 +601,8 @@ int rec_step(fw_batch *b, int p0) {
       if (b->disp) bl<R>(b, s, "flux_y", kb_flux_y<R, true, FluxYK<R>::K>, par, par);
       else bl<R>(b, s, "flux_y", kb_flux_y<R, false, FluxYK<R>::K>, par, par);
     }
-    bl<R>(b, s, "huv", kb_huv<R, P2>, par, par, alpha[st], beta[st]);
+    if (b->disp && !b->periodic) bl<R>(b, s, "huv", kb_huv<R, P2, true>, par, par, alpha[st], beta[st]);
+    else bl<R>(b, s, "huv", kb_huv<R, P2>, par, par, alpha[st], beta[st]);
     // line solves and the velocity tail beside UPDATE_MASK + WAVE_BREAKING
     BCHK(b, hipEventRecord(b->ev_fork, s));
     BCHK(b, hipStreamWaitEvent(sd, b->ev_fork, 0));

@@ -517,7 +517,11 @@ __device__ __forceinline__ void disp_terms_at(Dev<R> &d, const Geo &g, const Prm
   }
   // U4, V4, U1p, V1p are stored after the loads of the nonlinear terms (a
   // store would hold back the loads after it)
-  auto store4 = [&]() { d.U4[o] = r_u4; d.V4[o] = r_v4; d.U1p[o] = r_u1p; d.V1p[o] = r_v1p; };
+  auto store4 = [&]() {
+    d.U4[o] = r_u4;
+    d.V4[o] = r_v4;
+    if (!p.xd_in_huv) { d.U1p[o] = r_u1p; d.V1p[o] = r_v1p; }
+  };
   if (!(g2 > (R)0) || i < g.Ib || i > g.Ie || j < g.Jb || j > g.Je) {
     store4();
     return;
@@ -644,8 +648,9 @@ __global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_terms(Dev<R> d, Geo g
 // (a ghost cell holds the value of its mirror cell, negated for the arrays
 // odd in that direction, bc.F:306-529), and the terms of k_disp_terms are
 // formed from there.  The derivatives are not written to HBM and read back:
-// only what later routines read is stored (ETAT, ETAx, ETAy, the four cross
-// derivatives, U4, V4, U1p, V1p and the nonlinear sums).  Same operations on
+// only what later routines read is stored (ETAT, ETAx, ETAy, U4, V4, the
+// nonlinear sums, and the four cross derivatives and U1p, V1p unless the RK
+// update forms those itself, p.xd_in_huv).  Same operations on
 // the same operands as k_disp_deriv + ghost fill + k_disp_terms.
 constexpr int kDispTX = 64, kDispTY = 8;
 
@@ -689,18 +694,35 @@ __device__ __forceinline__ void disp_fused_body(Dev<R> d, Geo g, Prm<R> p, R DT,
   const Tile tb = xcd_tile(B);
   const int x0 = 1 + tb.x * kDispTX, y0 = 1 + tb.y * kDispTY;
   const int tx = threadIdx.x, ty = threadIdx.y;
-  for (int q = tx + ty * kDispTX; q < EX * EY; q += kDispTX * kDispTY) {
-    const int ex = q % EX, ey = q / EX;
-    const int i = x0 - 1 + ex, j = y0 - 1 + ey;
-    R v[8];
-    if (i >= 1 && i <= g.M && j >= 1 && j <= g.N) {
-      second_derivs<R, P2>(d, g, p, i, j, v);
-    } else {
+  // the ring of second derivatives: at most two cells per thread, both
+  // formed unconditionally (clamped into the grid, dropped outside it) so
+  // that their loads go out together
+  {
+    const int t = tx + ty * kDispTX;
+    constexpr int NT = kDispTX * kDispTY, NE = EX * EY;
+    static_assert(NE <= 2 * NT, "two ring cells per thread at most");
+    const int qa = t, qb = t + NT < NE ? t + NT : NE - 1;
+    auto ring = [&](int q, R *v, bool &in) {
+      const int ex = q % EX, ey = q / EX;
+      const int i = x0 - 1 + ex, j = y0 - 1 + ey;
+      in = i >= 1 && i <= g.M && j >= 1 && j <= g.N;
+      const int ic = i < 1 ? 1 : (i > g.M ? g.M : i), jc = j < 1 ? 1 : (j > g.N ? g.N : j);
+      second_derivs<R, P2>(d, g, p, ic, jc, v);
+    };
+    R va[8], vb[8];
+    bool ina, inb;
+    ring(qa, va, ina);
+    ring(qb, vb, inb);
+    {
+      const int ex = qa % EX, ey = qa / EX;
 #pragma unroll
-      for (int k = 0; k < 8; k++) v[k] = (R)0;
+      for (int k = 0; k < 8; k++) sS[k][ey][ex] = ina ? va[k] : (R)0;
     }
+    if (t + NT < NE) {
+      const int ex = qb % EX, ey = qb / EX;
 #pragma unroll
-    for (int k = 0; k < 8; k++) sS[k][ey][ex] = v[k];
+      for (int k = 0; k < 8; k++) sS[k][ey][ex] = inb ? vb[k] : (R)0;
+    }
   }
   __syncthreads();
   const int i = x0 + tx, j = y0 + ty;
@@ -1422,10 +1444,12 @@ __global__ void __launch_bounds__(256, 5) k_flux_y_march(Dev<R> d, Geo g, Prm<R>
 // One interior cell.  With keep, the MODULE GLOBAL outputs (SourceX/Y,
 // FrcInsX/Y, BreakSourceX/Y, WaveMaker_Mass in the band) are written; the
 // values ESTIMATE_HUV needs are returned either way (wm: the cell's
-// WaveMaker_Mass, new in the band, else the stored one).
-template <class R, bool P2>
+// WaveMaker_Mass, new in the band, else the stored one).  XD: U1p and V1p
+// of the cell are given (u1p_x, v1p_x) instead of read.
+template <class R, bool P2, bool XD = false>
 __device__ __forceinline__ void source_terms(const Dev<R> &d, const Geo &g, const Prm<R> &p, R AA, int i, int j,
-                                             bool keep, bool sums, R &sx_out, R &sy_out, R &wm_out) {
+                                             bool keep, bool sums, R &sx_out, R &sy_out, R &wm_out,
+                                             R u1p_x = (R)0, R v1p_x = (R)0) {
   const size_t o = IX(i, j), oe = IX(i + 1, j), ow = IX(i - 1, j), on = IX(i, j + 1), os = IX(i, j - 1);
   const Dv<R> dx = dvx<P2>(p), dy = dvy<P2>(p);
   const R mdf = p.MDF, g1 = p.G1, g2 = p.G2;
@@ -1444,7 +1468,14 @@ __device__ __forceinline__ void source_terms(const Dev<R> &d, const Geo &g, cons
   const R v4o = V4[o], v4e = V4[oe], v4w = V4[ow], v4n = V4[on], v4s = V4[os];
   const R ue = U[oe], uw = U[ow], un = U[on], us = U[os];
   const R ve = V[oe], vw = V[ow], vn = V[on], vs = V[os];
-  const R u1p = d.U1p[o], v1p = d.V1p[o];
+  R u1p, v1p;
+  if constexpr (XD) {
+    u1p = u1p_x;
+    v1p = v1p_x;
+  } else {
+    u1p = d.U1p[o];
+    v1p = d.V1p[o];
+  }
   // (U1pp + U2) + U3 (sources.F:254): with every intermediate kept, from the
   // MODULE GLOBAL arrays; otherwise the sums CAL_DISPERSION's kernel formed
   const R *su[3] = {sums ? d.sumU : d.U1pp, sums ? d.sumU : d.U2, sums ? d.sumU : d.U3};
@@ -1559,11 +1590,12 @@ __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
 // read back, which is the same arithmetic on the same inputs: SourceTerms
 // reads nothing ESTIMATE_HUV writes.
 // XD: the four cross derivatives Uxy, Vxy, DUxy, DVxy, which only the line
-// coefficients of this cell read, are formed here from U, V, Depth and MASK9
-// (unchanged since CAL_DISPERSION) instead of read from the planes the
-// dispersion kernel would otherwise write: the same operations on the same
-// operands as there (an interior cell is its own mirror cell; wall cells
-// take 0).
+// coefficients of this cell read, and U1p, V1p, which only its source terms
+// read, are formed here from U, V, Depth and MASK9 (unchanged since
+// CAL_DISPERSION) instead of read from the planes the dispersion kernel
+// would otherwise write: the same operations on the same operands as there
+// (an interior cell is its own mirror cell; wall cells take 0 for the cross
+// derivatives).
 template <class R, bool FUSED, bool P2, bool XD = false>
 __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA, const Blk &B) {
   const Tile tb = xcd_tile(B);
@@ -1593,7 +1625,7 @@ __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, 
     const R m9 = (R)(disp ? d.MASK9 : d.MASK)[o];
     const R dxl = d.Depth[disp ? IX(i - 1, j) : o], dxr = d.Depth[disp ? IX(i + 1, j) : o];
     const R dyl = d.Depth[disp ? IX(i, j - 1) : o], dyr = d.Depth[disp ? IX(i, j + 1) : o];
-    R vxy, dvxy, uxy, duxy;
+    R vxy, dvxy, uxy, duxy, u1p = (R)0, v1p = (R)0;
     if constexpr (XD) {
       const R *U = d.U, *V = d.V, *Dp = d.Depth;
       auto DU = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * U[q]; };
@@ -1606,6 +1638,12 @@ __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, 
       vxy = wall ? (R)0 : vxy_s;
       duxy = wall ? (R)0 : duxy_s;
       dvxy = wall ? (R)0 : dvxy_s;
+      // U1p, V1p (dispersion.F:293-330) as k_disp_terms forms them
+      const R uxx = dfxx(g, U, i, j, dx) * m9, vyy = dfyy(g, V, i, j, dy) * m9;
+      const R duxx = gfxx<R>(DU, i, j, dx) * m9, dvyy = gfyy<R>(DV, i, j, dy) * m9;
+      const R B1 = p.B1;
+      u1p = (R)0.5 * ((R)1 - B1) * ((R)1 - B1) * depth * depth * (uxx + vxy) + (B1 - (R)1) * depth * (duxx + dvxy);
+      v1p = (R)0.5 * ((R)1 - B1) * ((R)1 - B1) * depth * depth * (uxy + vyy) + (B1 - (R)1) * depth * (duxy + dvyy);
     } else {
       vxy = (disp ? d.Vxy : d.Depth)[o];
       dvxy = (disp ? d.DVxy : d.Depth)[o];
@@ -1614,7 +1652,7 @@ __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, 
     }
     R srcx, srcy, wmm;
     if (FUSED) {
-      source_terms<R, P2>(d, g, p, AA, i, j, p.full_state != 0, p.full_state == 0, srcx, srcy, wmm);
+      source_terms<R, P2, XD>(d, g, p, AA, i, j, p.full_state != 0, p.full_state == 0, srcx, srcy, wmm, u1p, v1p);
     } else {
       srcx = d.SourceX[o];
       srcy = d.SourceY[o];

@@ -147,7 +147,9 @@ int fw_reduce_stats_noalloc(fw_ctx *ctx, fw_stats *out);
  * roller flux and undertow ROLLER_FLUX, UNDERTOW_U, UNDERTOW_V), for
  * routine-by-routine comparison with the reference.  Default 0: only the
  * arrays a later routine reads are written (U1pp+U2+U3 and V1pp+V2+V3 only
- * as the sums SourceTerms forms); results are identical.  fw_source_terms
+ * as the sums SourceTerms forms; in fw_step, whose RK update forms the
+ * cross derivatives Uxy, Vxy, DUxy, DVxy and U1p, V1p of each cell itself,
+ * not those either); results are identical.  fw_source_terms
  * always writes its outputs; the face states, the dispersion first
  * derivatives and nonlinear terms and the mean-flow diagnostics follow this
  * option in the per-routine entry points too (without it, fw_source_terms

@@ -12,7 +12,7 @@ from common import bitwise_mismatch
 pytestmark = pytest.mark.gpu
 
 KEYS = ["Eta", "Eta0", "U", "V", "U0", "V0", "HU", "HV", "Ubar", "Vbar", "Ubar0", "Vbar0", "H", "MASK", "MASK9",
-        "AGE_BREAKING", "nu_break", "ETAT", "ETAx", "ETAy", "U4", "V4", "U1p", "V1p", "DepthX", "DepthY", "P", "Q",
+        "AGE_BREAKING", "nu_break", "ETAT", "ETAx", "ETAy", "U4", "V4", "DepthX", "DepthY", "P", "Q",
         "Fx", "Fy", "Gx", "Gy"]
 
 

@@ -48,10 +48,10 @@ DISP_SUMMED = {"U1pp", "V1pp", "U2", "V2", "U3", "V3"}
 # which ESTIMATE_HUV reads, are stored)
 DISP_SECOND = {"Uxx", "DUxx", "Vyy", "DVyy"}
 PRODUCTION_SKIP = FACE_STATES | DISP_FIRST | SOURCES | DIAGNOSTICS | DISP_SUMMED | DISP_SECOND
-# and, in fw_step's production path, the cross derivatives, which the RK
-# update forms itself at its own cell (the one-pass dispersion does not
-# store them; the per-routine CAL_DISPERSION does)
-DISP_CROSS = {"Uxy", "Vxy", "DUxy", "DVxy"}
+# and, in fw_step's production path, the cross derivatives and U1p, V1p,
+# which the RK update forms itself at its own cell (the one-pass dispersion
+# does not store them; the per-routine CAL_DISPERSION does)
+DISP_CROSS = {"Uxy", "Vxy", "DUxy", "DVxy", "U1p", "V1p"}
 
 
 def _compare(g, o, where, full_state=True, step=False):

@@ -24,8 +24,8 @@ pytestmark = [pytest.mark.gpu,
 
 # arrays the production path keeps up to date after whole steps
 PRODUCTION_KEYS = ["Eta", "Eta0", "U", "V", "U0", "V0", "HU", "HV", "Ubar", "Vbar", "Ubar0", "Vbar0", "H",
-                   "MASK", "MASK9", "AGE_BREAKING", "nu_break", "ETAT", "ETAx", "ETAy", "U4", "V4", "U1p",
-                   "V1p", "DepthX", "DepthY", "P", "Q", "Fx", "Fy", "Gx", "Gy"]
+                   "MASK", "MASK9", "AGE_BREAKING", "nu_break", "ETAT", "ETAx", "ETAy", "U4", "V4",
+                   "DepthX", "DepthY", "P", "Q", "Fx", "Fy", "Gx", "Gy"]
 
 
 def _compare(g, o, keys, where):
