@@ -833,6 +833,27 @@ __device__ __forceinline__ R din_val_ref(R vm2, R vm1, R v0, R vp1, int mm2, int
   return din_from(t1, t2, t3, mm2 == 0 || mp1 == 0, mm1 == 0 || m0 == 0,
                   [](R a, R b, R c) { return mm3(a, b, c); });
 }
+// FP32 form of the same.  mm3(a, b, c) is the median of min(a,2b,2c),
+// max(a,2b,2c) and 0: for a >= 0 the maximum is >= 0, so the median is the
+// minimum when that is >= 0 and 0 otherwise (= max(0, min)); for a < 0 the
+// minimum is < 0 and the median is min(0, max) the same way.  One v_med3
+// after v_min3/v_max3, equal to mm3 up to the sign of a zero like mm3_hw.
+// The edge and dry cases need no limited slopes: next to a dry cell
+// d1 = d2 = d3 = mm3(t2, t2, t2) = t2 (for t2 != -0), so the sum is
+// (t2 - 2 t2) + t2 from the same operations; in a dry cell d1..d3 = +0 and
+// the sum is +0.
+__device__ __forceinline__ float mm3_med(float a, float tb, float tc) {
+  return __builtin_amdgcn_fmed3f(hmin(hmin(a, tb), tc), hmax(hmax(a, tb), tc), 0.0f);
+}
+__device__ __forceinline__ float din_from_med(float t1, float t2, float t3, bool edge, bool dry) {
+  const float tb1 = 2.0f * t1, tb2 = 2.0f * t2, tb3 = 2.0f * t3;
+  const float d1 = mm3_med(t1, tb2, tb3), d2 = mm3_med(t2, tb3, tb1), d3 = mm3_med(t3, tb1, tb2);
+  float X = d3 - 2.0f * d2 + d1;
+  const float Xe = t2 - tb2 + t2;
+  X = edge ? Xe : X;
+  X = dry ? 0.0f : X;
+  return t2 - 1.0f / 6.0f * X;
+}
 // Din on the hardware minimum/maximum; sets `exact` for a lane that needs
 // din_val_ref instead (t2 = -0, not dry)
 template <class R>
@@ -840,7 +861,8 @@ __device__ __forceinline__ R din_val_hw(R vm2, R vm1, R v0, R vp1, int mm2, int 
   const R t1 = vm1 - vm2, t2 = v0 - vm1, t3 = vp1 - v0;
   const bool dry = mm1 == 0 || m0 == 0;
   exact = exact || (neg_zero(t2) && !dry);
-  return din_from(t1, t2, t3, mm2 == 0 || mp1 == 0, dry, [](R a, R b, R c) { return mm3_hw(a, b, c); });
+  if constexpr (sizeof(R) == 4) return din_from_med(t1, t2, t3, mm2 == 0 || mp1 == 0, dry);
+  else return din_from(t1, t2, t3, mm2 == 0 || mp1 == 0, dry, [](R a, R b, R c) { return mm3_hw(a, b, c); });
 }
 template <class R>
 __device__ __forceinline__ R din_val(R vm2, R vm1, R v0, R vp1, int mm2, int mm1, int m0, int mp1) {
@@ -866,22 +888,38 @@ __device__ __forceinline__ float mdiv(float a, float b) {
   return __builtin_copysignf(q1, q);
 }
 __device__ __forceinline__ double mdiv(double a, double b) { return a / b; }
-// The van Leer weights with mdiv: with |a|, |b| in [SMALL, 2^30] (SMALL is
-// about 2^-19.9) the ratios lie in 2^-50..2^50 and every numerator and
-// denominator in 2^-50..2^51 (or is 0, or a denominator 1 + rat is 0: the
-// quotient is then discarded as in the reference), inside mdiv's range.
-// `exact` is set for a lane outside it, which must use vanleer instead.
+// mdiv without the sign fix, for a numerator that is nonzero (the quotient is
+// then correctly rounded, sign included) or +0: then q = +0*y has y's sign,
+// r = -b*q + 0 = +0 and q' = +0*y + q = q (zeros of one sign).
+__device__ __forceinline__ float mdiv_p(float a, float b) {
+  const float y0 = __builtin_amdgcn_rcpf(b);
+  const float y = __builtin_fmaf(__builtin_fmaf(-b, y0, 1.0f), y0, y0);
+  const float q = a * y;
+  return __builtin_fmaf(__builtin_fmaf(-b, q, a), y, q);
+}
+// The van Leer weights with mdiv (FP32): with |a|, |b| in [SMALL, 2^30]
+// (SMALL is about 2^-19.9) the ratios lie in 2^-50..2^50 and every
+// numerator and denominator in 2^-50..2^51 (or is 0, or a denominator
+// 1 + rat is 0: the quotient is then discarded as in the reference), inside
+// mdiv's range.  `exact` is set for a lane outside it (or a NaN operand),
+// which must use vanleer instead.  The range test on the operands before
+// the SMALL clamp is the same test (SMALL < 2^30).  The clamp
+// |a| <= SMALL ? SMALL*SIGN(1,a) : a is copysign(med3(|a|, SMALL, 2^30), a)
+// for every a of the range (one v_med3, which needs no canonical operand as
+// a maximum would).  The numerators are b, a (|.| >= SMALL) and rat + |rat|
+// (+0 or positive), so mdiv_p applies.
 template <class R>
 __device__ __forceinline__ void vanleer_m(R a, R b, R &v1, R &v2, R &ta, R &tb, bool &exact) {
-  a = dabs(a) <= SMALL ? copysign_(SMALL, a) : a;
-  b = dabs(b) <= SMALL ? copysign_(SMALL, b) : b;
-  exact = exact || !(dabs(a) <= (R)1073741824 && dabs(b) <= (R)1073741824);
-  const R rat = mdiv(b, a);
-  const R q1 = mdiv(rat + dabs(rat), (R)1 + rat);
-  v1 = dabs((R)1 + rat) > SMALL ? q1 : (R)0;
-  const R rat2 = mdiv(a, b);
-  const R q2 = mdiv(rat2 + dabs(rat2), (R)1 + rat2);
-  v2 = dabs((R)1 + rat2) > SMALL ? q2 : (R)0;
+  static_assert(sizeof(R) == 4, "FP32 only");
+  exact = exact || !(dabs(a) <= 1073741824.0f && dabs(b) <= 1073741824.0f);
+  a = copysign_(__builtin_amdgcn_fmed3f(dabs(a), SMALL, 1073741824.0f), a);
+  b = copysign_(__builtin_amdgcn_fmed3f(dabs(b), SMALL, 1073741824.0f), b);
+  const float rat = mdiv_p(b, a);
+  const float q1 = mdiv_p(rat + dabs(rat), 1.0f + rat);
+  v1 = dabs(1.0f + rat) > SMALL ? q1 : 0.0f;
+  const float rat2 = mdiv_p(a, b);
+  const float q2 = mdiv_p(rat2 + dabs(rat2), 1.0f + rat2);
+  v2 = dabs(1.0f + rat2) > SMALL ? q2 : 0.0f;
   ta = a;
   tb = b;
 }
@@ -992,8 +1030,12 @@ template <class R>
 __device__ __forceinline__ void pair_states_m(R vc, R D, R Dn, R &Lnext, R &Rc, bool &exact) {
   R w1, w2, a, b;
   vanleer_m(D, Dn, w1, w2, a, b, exact);
-  Lnext = vc + ((R)1 / (R)6) * (w1 * a + (R)2 * w2 * b);
-  Rc = vc - ((R)1 / (R)6) * ((R)2 * w1 * a + w2 * b);
+  // where vanleer_m's range holds, w1*a and w2*b are 0 or normal numbers
+  // below 2^31 (w in 2^-49..2, |a|, |b| in SMALL..2^30), so (2*w2)*b is
+  // 2*(w2*b) exactly, and w1*a + 2*(w2*b) is one fma, zero signs included
+  const R p1 = w1 * a, p2 = w2 * b;
+  Lnext = vc + ((R)1 / (R)6) * __builtin_fmaf(p2, 2.0f, p1);
+  Rc = vc - ((R)1 / (R)6) * __builtin_fmaf(p1, 2.0f, p2);
 }
 
 // left/right states of face c along x for field v: lanes with dok hold a
