@@ -41,15 +41,16 @@ KERNEL_TRAFFIC = {
     "disp_deriv": (6, 1, 11, "cell"),    # U V Depth Eta P Q, MASK9 -> 8 second derivatives, ETAT ETAx ETAy
     "disp_terms": (17, 1, 6, "cell"),    # 8 second derivatives, ETAT ETAx ETAy, U V U0 V0 Depth Eta
                                          #   -> U4 V4 U1p V1p and the sums U1pp+U2+U3, V1pp+V2+V3
-    "disp": (8, 1, 13, "cell"),          # one pass: U V U0 V0 Depth Eta P Q, MASK9 -> ETAT ETAx ETAy,
-                                         #   Uxy Vxy DUxy DVxy, U4 V4 U1p V1p and the two sums
+    "disp": (8, 1, 7, "cell"),           # one pass: U V U0 V0 Depth Eta P Q, MASK9 -> ETAT ETAx ETAy,
+                                         #   U4 V4 and the two sums (the RK update forms the cross
+                                         #   derivatives and U1p, V1p of its cell itself)
     "sources": (24, 2, 2, "inner"),
     "huv": (24, 2, 12, "cell"),
-    # SourceTerms fused into ESTIMATE_HUV: U V U4 V4 Cd H P Q Eta DepthX DepthY
-    # U1p V1p, the two dispersive sums, nu_break HU HV Eta0 Ubar0 Vbar0 Ubar
-    # Vbar Fx Fy Gx Gy Depth Uxy DUxy Vxy DVxy, MASK MASK9
-    #   -> Eta Ubar Vbar H and the six line-solve coefficient arrays
-    "src_huv": (32, 2, 10, "cell"),
+    # SourceTerms fused into ESTIMATE_HUV: U V U4 V4 Cd H P Q Eta DepthX DepthY,
+    # the two dispersive sums, nu_break HU HV Eta0 Ubar0 Vbar0 Ubar Vbar Fx Fy
+    # Gx Gy Depth, MASK MASK9 (the cross derivatives and U1p, V1p formed from
+    # U, V, Depth) -> Eta Ubar Vbar H and the six line-solve coefficient arrays
+    "src_huv": (26, 2, 10, "cell"),
     "trid": (6, 0, 2, "inner"),
     "uv_final": (3, 1, 4, "cell"),
     "mask": (4, 2, 2, "cell"),
@@ -485,8 +486,8 @@ def main():
                                f"waves, {n}x{n} cells dx=2 m, sponges W/S/N, wetting-drying, "
                                f"dispersion + breaking, whole RK3 steps",
                    "Mglob": n, "Nglob": n, "precision": "FP64" if rb == 8 else "FP32",
-                   "line_solver": "thomas, c' recurrence run ahead on a second stream (bitwise the "
-                                  "reference's recurrence)",
+                   "line_solver": "thomas sweep, c' and d' recurrences on two waves of one workgroup "
+                                  "(LDS-staged; bitwise the reference's recurrence)",
                    "froude_cap_math": "host C library" if args.host_math else "device math library",
                    "parallelism": "one domain on one GPU"},
         # The whole step against its canonical algorithmic traffic:

@@ -13,6 +13,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 #include "fw_fields.h"
 
@@ -339,24 +340,67 @@ __global__ void k_copy3(R *d0, const R *s0, R *d1, const R *s1, R *d2, const R *
 // (derivatives.F:123-289) on the interior, ETAT on 1..Mloc-1 x 1..Nloc-1,
 // Ut/Vt on all cells, and zeroed cross derivatives on the wall lines.
 // DU = max(Depth,MinDepthFrc) U and DUt are recomputed at each stencil point.
+//
+// A stencil's cells are addressed from its centre row and the rows above and
+// below as 32-bit byte offsets (C32): with the array's base in scalar
+// registers a load then takes the base + 32-bit offset + immediate form, so
+// every array read at a stencil point shares that row's offset (the
+// columns are immediates, and neighbouring columns merge into one wide
+// load) instead of forming a 64-bit address per array and point.  Used
+// around cells of the interior and their mirror cells (every offset >= 0),
+// for arrays of the working precision on planes below 2^32 bytes.
+struct C32 {
+  uint32_t bm, b0, bp;  // byte offsets of rows j-1, j, j+1 at column i
+  template <class T>
+  __device__ __forceinline__ const T &at(const T *a, int di, int dj) const {
+    const uint32_t b = dj < 0 ? bm : (dj > 0 ? bp : b0);
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(a) + b + di * (int)sizeof(T));
+  }
+};
+template <class R>
+__device__ __forceinline__ C32 c32(const Geo &g, int i, int j) {
+  const uint32_t b0 = ((uint32_t)(j - 1) * (uint32_t)g.P + (uint32_t)(i - 1)) * (uint32_t)sizeof(R);
+  const uint32_t p = (uint32_t)g.P * (uint32_t)sizeof(R);
+  return C32{b0 - p, b0, b0 + p};
+}
 template <class R>
 __device__ __forceinline__ R dfx(const Geo &g, const R *a, int i, int j, const Dv<R> &dx) {
-  return half(dv(a[IX(i + 1, j)] - a[IX(i - 1, j)], dx));
+  const C32 c = c32<R>(g, i, j);
+  return half(dv(c.at(a, 1, 0) - c.at(a, -1, 0), dx));
 }
 template <class R>
 __device__ __forceinline__ R dfy(const Geo &g, const R *a, int i, int j, const Dv<R> &dy) {
-  return half(dv(a[IX(i, j + 1)] - a[IX(i, j - 1)], dy));
+  const C32 c = c32<R>(g, i, j);
+  return half(dv(c.at(a, 0, 1) - c.at(a, 0, -1), dy));
 }
 template <class R>
 __device__ __forceinline__ R dfxx(const Geo &g, const R *a, int i, int j, const Dv<R> &dx) {
-  return dv(dv(a[IX(i + 1, j)] - (R)2 * a[IX(i, j)] + a[IX(i - 1, j)], dx), dx);
+  const C32 c = c32<R>(g, i, j);
+  return dv(dv(c.at(a, 1, 0) - (R)2 * c.at(a, 0, 0) + c.at(a, -1, 0), dx), dx);
 }
 template <class R>
 __device__ __forceinline__ R dfyy(const Geo &g, const R *a, int i, int j, const Dv<R> &dy) {
-  return dv(dv(a[IX(i, j + 1)] - (R)2 * a[IX(i, j)] + a[IX(i, j - 1)], dy), dy);
+  const C32 c = c32<R>(g, i, j);
+  return dv(dv(c.at(a, 0, 1) - (R)2 * c.at(a, 0, 0) + c.at(a, 0, -1), dy), dy);
 }
 template <class R>
 __device__ __forceinline__ R dfxy(const Geo &g, const R *a, int i, int j, const Dv<R> &dx, const Dv<R> &dy) {
+  const C32 c = c32<R>(g, i, j);
+  R t1 = dv(half(c.at(a, 1, 1) - c.at(a, 1, -1)), dy);
+  R t2 = dv(half(c.at(a, -1, 1) - c.at(a, -1, -1)), dy);
+  return dv(half(t1 - t2), dx);
+}
+// the same with 64-bit addresses (for a kernel with no registers to spare)
+template <class R>
+__device__ __forceinline__ R dfxx_ix(const Geo &g, const R *a, int i, int j, const Dv<R> &dx) {
+  return dv(dv(a[IX(i + 1, j)] - (R)2 * a[IX(i, j)] + a[IX(i - 1, j)], dx), dx);
+}
+template <class R>
+__device__ __forceinline__ R dfyy_ix(const Geo &g, const R *a, int i, int j, const Dv<R> &dy) {
+  return dv(dv(a[IX(i, j + 1)] - (R)2 * a[IX(i, j)] + a[IX(i, j - 1)], dy), dy);
+}
+template <class R>
+__device__ __forceinline__ R dfxy_ix(const Geo &g, const R *a, int i, int j, const Dv<R> &dx, const Dv<R> &dy) {
   R t1 = dv(half(a[IX(i + 1, j + 1)] - a[IX(i + 1, j - 1)]), dy);
   R t2 = dv(half(a[IX(i - 1, j + 1)] - a[IX(i - 1, j - 1)]), dy);
   return dv(half(t1 - t2), dx);
@@ -422,12 +466,18 @@ __device__ __forceinline__ void disp_deriv_body(Dev<R> d, Geo g, Prm<R> p, R DT,
   }
   const R m9 = (R)d.MASK9[o];
   const R *U = d.U, *V = d.V, *Dp = d.Depth, *U0 = d.U0, *V0 = d.V0;
-  auto DU = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * U[q]; };
-  auto DV = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * V[q]; };
-  auto UT = [&](int a, int b) { size_t q = IX(a, b); return ddiv(U[q] - U0[q], DT); };
-  auto VT = [&](int a, int b) { size_t q = IX(a, b); return ddiv(V[q] - V0[q], DT); };
-  auto DUT = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * ddiv(U[q] - U0[q], DT); };
-  auto DVT = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * ddiv(V[q] - V0[q], DT); };
+  const int ci = i, cj = j;
+  const C32 cc = c32<R>(g, i, j);
+  auto DU = [&](int a, int b) { return rmax(cc.at(Dp, a - ci, b - cj), mdf) * cc.at(U, a - ci, b - cj); };
+  auto DV = [&](int a, int b) { return rmax(cc.at(Dp, a - ci, b - cj), mdf) * cc.at(V, a - ci, b - cj); };
+  auto UT = [&](int a, int b) { return ddiv(cc.at(U, a - ci, b - cj) - cc.at(U0, a - ci, b - cj), DT); };
+  auto VT = [&](int a, int b) { return ddiv(cc.at(V, a - ci, b - cj) - cc.at(V0, a - ci, b - cj), DT); };
+  auto DUT = [&](int a, int b) {
+    return rmax(cc.at(Dp, a - ci, b - cj), mdf) * ddiv(cc.at(U, a - ci, b - cj) - cc.at(U0, a - ci, b - cj), DT);
+  };
+  auto DVT = [&](int a, int b) {
+    return rmax(cc.at(Dp, a - ci, b - cj), mdf) * ddiv(cc.at(V, a - ci, b - cj) - cc.at(V0, a - ci, b - cj), DT);
+  };
   const bool wall = (i == g.Ib || i == g.Ie || j == g.Jb || j == g.Je);
   const R uxx = dfxx(g, U, i, j, dx) * m9;
   // (the cross derivatives are formed on wall cells too and then dropped:
@@ -488,8 +538,12 @@ __global__ void __launch_bounds__(WideLB<R>::wg512) k_disp_deriv(Dev<R> d, Geo g
 // Uxx, 1 Vxy, 2 Uxy, 3 Vyy, 4 DUxx, 5 DVxy, 6 DUxy, 7 DVyy) at (i+di, j+dj)
 // after their ghost fill, and 8, 9, 10 give ETAx, ETAy, ETAT at (i, j): from
 // the arrays (k_disp_terms) or from the fused kernel's tile (k_disp_fused).
-template <class R, bool P2, class SD>
-__device__ __forceinline__ void disp_terms_at(Dev<R> &d, const Geo &g, const Prm<R> &p, R DT, int i, int j, SD S) {
+// TT(di, dj, k) gives max(Depth, MinDepthFrc) (k 0), Ut = (U-U0)/DT (k 1)
+// and Vt (k 2) at (i+di, j+dj), or nothing (the terms form them here).
+struct NoTT {};
+template <class R, bool P2, class SD, class TT = NoTT>
+__device__ __forceinline__ void disp_terms_at(Dev<R> &d, const Geo &g, const Prm<R> &p, R DT, int i, int j, SD S,
+                                              TT T = TT{}) {
   size_t o = IX(i, j);
   const R B1 = p.B1, B2 = p.B2, b2 = p.b2, g2 = p.G2;
   const Dv<R> dx = dvx<P2>(p), dy = dvy<P2>(p);
@@ -531,8 +585,10 @@ __device__ __forceinline__ void disp_terms_at(Dev<R> &d, const Geo &g, const Prm
   const R m9 = (R)d.MASK9[o], mdf = p.MDF;
   const bool wall = (i == g.Ib || i == g.Ie || j == g.Jb || j == g.Je);
   const R *Uv = d.U, *Vv = d.V, *Dp = d.Depth, *U0 = d.U0, *V0 = d.V0;
-  auto DU = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * Uv[q]; };
-  auto DV = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * Vv[q]; };
+  const int ci = i, cj = j;
+  const C32 cc = c32<R>(g, i, j);
+  auto DU = [&](int a, int b) { return rmax(cc.at(Dp, a - ci, b - cj), mdf) * cc.at(Uv, a - ci, b - cj); };
+  auto DV = [&](int a, int b) { return rmax(cc.at(Dp, a - ci, b - cj), mdf) * cc.at(Vv, a - ci, b - cj); };
   // Ut = (U-U0)/DT and max(Depth,MinDepthFrc) once per stencil point
   // (Ut is not needed at (i,j+-1), Vt not at (i+-1,j)); DUt = hm*Ut
   R hm[3][3], ut[3][3], vt[3][3];
@@ -540,10 +596,15 @@ __device__ __forceinline__ void disp_terms_at(Dev<R> &d, const Geo &g, const Prm
   for (int dj = -1; dj <= 1; dj++) {
 #pragma unroll
     for (int di = -1; di <= 1; di++) {
-      const size_t q = IX(i + di, j + dj);
-      hm[dj + 1][di + 1] = rmax(Dp[q], mdf);
-      ut[dj + 1][di + 1] = (dj == 0 || di != 0) ? ddiv(Uv[q] - U0[q], DT) : (R)0;
-      vt[dj + 1][di + 1] = (di == 0 || dj != 0) ? ddiv(Vv[q] - V0[q], DT) : (R)0;
+      if constexpr (std::is_same<TT, NoTT>::value) {
+        hm[dj + 1][di + 1] = rmax(cc.at(Dp, di, dj), mdf);
+        ut[dj + 1][di + 1] = (dj == 0 || di != 0) ? ddiv(cc.at(Uv, di, dj) - cc.at(U0, di, dj), DT) : (R)0;
+        vt[dj + 1][di + 1] = (di == 0 || dj != 0) ? ddiv(cc.at(Vv, di, dj) - cc.at(V0, di, dj), DT) : (R)0;
+      } else {
+        hm[dj + 1][di + 1] = T(di, dj, 0);
+        ut[dj + 1][di + 1] = T(di, dj, 1);
+        vt[dj + 1][di + 1] = T(di, dj, 2);
+      }
     }
   }
   auto UT = [&](int a, int b) { return ut[b - j + 1][a - i + 1]; };
@@ -561,7 +622,7 @@ __device__ __forceinline__ void disp_terms_at(Dev<R> &d, const Geo &g, const Prm
   const R f_DUtxx = gfxx<R>(DUT, i, j, dx) * m9, f_DVtyy = gfyy<R>(DVT, i, j, dy) * m9;
   const R f_DUtxy = wall ? (R)0 : gfxy<R>(DUT, i, j, dx, dy) * m9;
   const R f_DVtxy = wall ? (R)0 : gfxy<R>(DVT, i, j, dx, dy) * m9;
-#define A_(n, a, b) d.n[IX(a, b)]
+#define A_(n, a, b) cc.at(d.n, (a) - i, (b) - j)
   R UxxVxy = S(0, 0, 0) + S(1, 0, 0);
   R UxyVyy = S(2, 0, 0) + S(3, 0, 0);
   R UxxVxy_x = dv(half(S(0, 1, 0) + S(1, 1, 0) - S(0, -1, 0) - S(1, -1, 0)), dx);
@@ -668,8 +729,10 @@ __device__ __forceinline__ void second_derivs(const Dev<R> &d, const Geo &g, con
   if (j > g.Je) { sj = 2 * g.Je + 1 - j; ny = true; }
   const R m9 = (R)d.MASK9[IX(si, sj)];
   const R *U = d.U, *V = d.V, *Dp = d.Depth;
-  auto DU = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * U[q]; };
-  auto DV = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * V[q]; };
+  const int ci = si, cj = sj;
+  const C32 cc = c32<R>(g, si, sj);
+  auto DU = [&](int a, int b) { return rmax(cc.at(Dp, a - ci, b - cj), mdf) * cc.at(U, a - ci, b - cj); };
+  auto DV = [&](int a, int b) { return rmax(cc.at(Dp, a - ci, b - cj), mdf) * cc.at(V, a - ci, b - cj); };
   const bool wall = (si == g.Ib || si == g.Ie || sj == g.Jb || sj == g.Je);
   const R uxx = dfxx(g, U, si, sj, dx) * m9;
   const R uxy_s = dfxy(g, U, si, sj, dx, dy) * m9, vxy_s = dfxy(g, V, si, sj, dx, dy) * m9;
@@ -691,6 +754,9 @@ template <class R, bool P2>
 __device__ __forceinline__ void disp_fused_body(Dev<R> d, Geo g, Prm<R> p, R DT, const Blk &B) {
   constexpr int EX = kDispTX + 2, EY = kDispTY + 2;
   __shared__ R sS[8][EY][EX];
+  // max(Depth, MinDepthFrc), Ut and Vt of the tile and its ring, formed once
+  // per cell (the terms read them at the nine points of their stencil)
+  __shared__ R sT[3][EY][EX];
   const Tile tb = xcd_tile(B);
   const int x0 = 1 + tb.x * kDispTX, y0 = 1 + tb.y * kDispTY;
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -702,26 +768,35 @@ __device__ __forceinline__ void disp_fused_body(Dev<R> d, Geo g, Prm<R> p, R DT,
     constexpr int NT = kDispTX * kDispTY, NE = EX * EY;
     static_assert(NE <= 2 * NT, "two ring cells per thread at most");
     const int qa = t, qb = t + NT < NE ? t + NT : NE - 1;
-    auto ring = [&](int q, R *v, bool &in) {
+    const R mdf = p.MDF;
+    auto ring = [&](int q, R *v, R *tt, bool &in) {
       const int ex = q % EX, ey = q / EX;
       const int i = x0 - 1 + ex, j = y0 - 1 + ey;
       in = i >= 1 && i <= g.M && j >= 1 && j <= g.N;
       const int ic = i < 1 ? 1 : (i > g.M ? g.M : i), jc = j < 1 ? 1 : (j > g.N ? g.N : j);
       second_derivs<R, P2>(d, g, p, ic, jc, v);
+      const C32 c = c32<R>(g, ic, jc);
+      tt[0] = rmax(c.at(d.Depth, 0, 0), mdf);
+      tt[1] = ddiv(c.at(d.U, 0, 0) - c.at(d.U0, 0, 0), DT);
+      tt[2] = ddiv(c.at(d.V, 0, 0) - c.at(d.V0, 0, 0), DT);
     };
-    R va[8], vb[8];
+    R va[8], vb[8], ta[3], tb_[3];
     bool ina, inb;
-    ring(qa, va, ina);
-    ring(qb, vb, inb);
+    ring(qa, va, ta, ina);
+    ring(qb, vb, tb_, inb);
     {
       const int ex = qa % EX, ey = qa / EX;
 #pragma unroll
       for (int k = 0; k < 8; k++) sS[k][ey][ex] = ina ? va[k] : (R)0;
+#pragma unroll
+      for (int k = 0; k < 3; k++) sT[k][ey][ex] = ta[k];
     }
     if (t + NT < NE) {
       const int ex = qb % EX, ey = qb / EX;
 #pragma unroll
       for (int k = 0; k < 8; k++) sS[k][ey][ex] = inb ? vb[k] : (R)0;
+#pragma unroll
+      for (int k = 0; k < 3; k++) sT[k][ey][ex] = tb_[k];
     }
   }
   __syncthreads();
@@ -755,7 +830,8 @@ __device__ __forceinline__ void disp_fused_body(Dev<R> d, Geo g, Prm<R> p, R DT,
     return sS[k][ly + dj][lx + di];
   };
   const R uxy = sS[2][ly][lx], vxy = sS[1][ly][lx], duxy = sS[6][ly][lx], dvxy = sS[5][ly][lx];
-  disp_terms_at<R, P2>(d, g, p, DT, i, j, S);
+  auto T = [&](int di, int dj, int k) -> R { return sT[k][ly + dj][lx + di]; };
+  disp_terms_at<R, P2>(d, g, p, DT, i, j, S, T);
   if (has_etat) d.ETAT[o] = etat;
   if (!p.xd_in_huv) { d.Uxy[o] = uxy; d.Vxy[o] = vxy; d.DUxy[o] = duxy; d.DVxy[o] = dvxy; }
   if (inner ? (g2 || p.SHOW_BRK) : g2) {
@@ -1669,19 +1745,21 @@ __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, 
     const R dyl = d.Depth[disp ? IX(i, j - 1) : o], dyr = d.Depth[disp ? IX(i, j + 1) : o];
     R vxy, dvxy, uxy, duxy, u1p = (R)0, v1p = (R)0;
     if constexpr (XD) {
+      // (64-bit addresses here: this kernel is at its register limit, and
+      // the 32-bit row offsets of the stencil helpers cost it spills)
       const R *U = d.U, *V = d.V, *Dp = d.Depth;
       auto DU = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * U[q]; };
       auto DV = [&](int a, int b) { size_t q = IX(a, b); return rmax(Dp[q], mdf) * V[q]; };
       const Dv<R> dx = dvx<P2>(p), dy = dvy<P2>(p);
       const bool wall = (i == g.Ib || i == g.Ie || j == g.Jb || j == g.Je);
-      const R uxy_s = dfxy(g, U, i, j, dx, dy) * m9, vxy_s = dfxy(g, V, i, j, dx, dy) * m9;
+      const R uxy_s = dfxy_ix(g, U, i, j, dx, dy) * m9, vxy_s = dfxy_ix(g, V, i, j, dx, dy) * m9;
       const R duxy_s = gfxy<R>(DU, i, j, dx, dy) * m9, dvxy_s = gfxy<R>(DV, i, j, dx, dy) * m9;
       uxy = wall ? (R)0 : uxy_s;
       vxy = wall ? (R)0 : vxy_s;
       duxy = wall ? (R)0 : duxy_s;
       dvxy = wall ? (R)0 : dvxy_s;
       // U1p, V1p (dispersion.F:293-330) as k_disp_terms forms them
-      const R uxx = dfxx(g, U, i, j, dx) * m9, vyy = dfyy(g, V, i, j, dy) * m9;
+      const R uxx = dfxx_ix(g, U, i, j, dx) * m9, vyy = dfyy_ix(g, V, i, j, dy) * m9;
       const R duxx = gfxx<R>(DU, i, j, dx) * m9, dvyy = gfyy<R>(DV, i, j, dy) * m9;
       const R B1 = p.B1;
       u1p = (R)0.5 * ((R)1 - B1) * ((R)1 - B1) * depth * depth * (uxx + vxy) + (B1 - (R)1) * depth * (duxx + dvxy);
@@ -1864,8 +1942,19 @@ __global__ void k_trid(Dev<R> d, Geo g, Prm<R> p, int nxblk) {
 // results of all but the last chunk go back to the c, d arrays.
 template <class R>
 struct alignas(4 * sizeof(R)) TridRow {
-  R r, ca, da, keep;  // 1/a, c/a, d/a (or c, d when keep != 0)
+  R r, ca, da, keep;  // 1/a, c/a, d/a (or c, d where keep is set, keep_on)
 };
+// A kept row (the first, or A == 0) takes its c, d unchanged.  In FP32 the
+// flag is an all-ones bit mask and the choice one v_bfi_b32 on the serial
+// chain (a compare there would wait for VCC, which the division uses);
+// FP64 keeps 1 / 0 and a select.
+__device__ __forceinline__ float keep_on(float) { return __uint_as_float(0xffffffffu); }
+__device__ __forceinline__ double keep_on(double) { return 1.0; }
+__device__ __forceinline__ float keep_sel(float keep, float kept, float x) {
+  const uint32_t m = __float_as_uint(keep);
+  return __uint_as_float((__float_as_uint(kept) & m) | (__float_as_uint(x) & ~m));
+}
+__device__ __forceinline__ double keep_sel(double keep, double kept, double x) { return keep != 0.0 ? kept : x; }
 template <class R>
 struct alignas(2 * sizeof(R)) TridCD {
   R c, d;  // during elimination d first holds the denominator 1/a - c'_prev
@@ -1974,7 +2063,7 @@ __device__ __forceinline__ void trid_lds_body(Dev<R> d, Geo g, int nxblk, const 
         const R a = ra[q];
         TridRow<R> v;
         if (ch * CH + sk[q] == 0 || !(a != (R)0)) {
-          v.r = (R)0; v.ca = rc[q]; v.da = rd[q]; v.keep = (R)1;
+          v.r = (R)0; v.ca = rc[q]; v.da = rd[q]; v.keep = keep_on((R)0);
         } else {
           v.r = ddiv((R)1, a); v.ca = ddiv(rc[q], a); v.da = ddiv(rd[q], a); v.keep = (R)0;
         }
@@ -2028,7 +2117,7 @@ __device__ __forceinline__ void trid_lds_body(Dev<R> d, Geo g, int nxblk, const 
         auto step = [&](int k, const TridRow<R> &v) {
           const R den = v.r - cprev;
           R c = ddiv(v.ca, den);
-          if (v.keep != (R)0) c = v.ca;
+          c = keep_sel(v.keep, v.ca, c);
           out[k] = TridCD<R>{c, den};
           cprev = c;
         };
@@ -2044,7 +2133,7 @@ __device__ __forceinline__ void trid_lds_body(Dev<R> d, Geo g, int nxblk, const 
         TridCD<R> *io = T.o[(it - 1) % 3][lane];
         auto step = [&](int k, const TridDIn<R> &x) {
           R dd = ddiv(x.v.da - dprev, x.o.d);
-          if (x.v.keep != (R)0) dd = x.v.da;
+          dd = keep_sel(x.v.keep, x.v.da, dd);
           io[k].d = dd;
           dprev = dd;
         };

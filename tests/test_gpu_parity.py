@@ -262,6 +262,54 @@ def test_full_size_beach_step_matches_oracle(prec, gpu_available):
     g.close()
 
 
+@pytest.mark.parametrize("prec", ["sp", "dp"])
+@pytest.mark.parametrize("name", ["tiny2d", "ragged2d"])
+def test_fused_routine_and_upload_interleaved(name, prec, gpu_available):
+    """Production fw_step mixed with the per-routine entry points and an
+    upload: fused steps while the wet/dry front moves, a step routine by
+    routine, fused steps, an upload of a changed MASK9 (which the line
+    coefficients depend on), fused steps: every array the production path
+    keeps, bit for bit against the oracle."""
+    import cases
+    import fwgpu
+    import fwsetup
+    import oracle
+    print(f"\n[fused, routine and upload interleaved] case={name} {prec}", flush=True)
+    if name == "ragged2d":
+        st = fwsetup.build_state(cases.ragged2d(), prec)
+    else:
+        st, _ = load_golden(name, prec)
+    g = fwgpu.Solver(st, prec)
+    o = oracle.OracleSolver(st)
+    cmp = lambda where: _compare(g, o, where, full_state=False, step=True)  # noqa: E731
+    masks = [o["MASK"].copy()]
+    for n in (1, 4, 15):
+        gdt, gt = g.step(n)
+        o.run(n)
+        assert (gdt, gt) == (o.dt.value, o.time.value)
+        cmp(f"after fw_step({n})")
+        masks.append(o["MASK"].copy())
+    assert any((a != b).any() for a, b in zip(masks, masks[1:])), "no wet/dry change in the fused steps"
+    _routine_steps(g, o, st, cmp, 1)
+    for n in (1, 2):
+        gdt, gt = g.step(n)
+        o.run(n)
+        assert (gdt, gt) == (o.dt.value, o.time.value)
+        cmp(f"after routines, fw_step({n})")
+    m9 = o["MASK9"].copy()
+    wet = np.argwhere(m9 == 1)
+    sel = wet[:: max(1, len(wet) // 40)]
+    m9[sel[:, 0], sel[:, 1]] = 0
+    o["MASK9"][...] = m9
+    g.upload("MASK9", m9)
+    for n in (1, 2):
+        gdt, gt = g.step(n)
+        o.run(n)
+        assert (gdt, gt) == (o.dt.value, o.time.value)
+        cmp(f"after a MASK9 upload, fw_step({n})")
+    g.close()
+
+
 TOL = {"sp": 1e-5, "dp": 1e-10}
 
 
