@@ -1456,8 +1456,9 @@ __global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo 
 // row loads); no LDS, no barrier; the strip's two end slopes and one pair
 // are formed again by the neighbouring strip.  Same operations on the same
 // operands as the cell-by-cell reconstruction.
-// (five waves per SIMD: 96 VGPRs with a few spilled dwords beat four waves
-// at 103)
+// (four waves per SIMD: with the slopes' -0 fallback taken in one branch
+// for all fields, five waves spill and lose 20 %; four are 2 % faster than
+// the per-field branches at five)
 template <class R, bool DISP, int RS>
 __device__ __forceinline__ void flux_y_march_body(Dev<R> d, Geo g, Prm<R> p, const Blk &B) {
   constexpr int NV = DISP ? 7 : 5;
@@ -1498,11 +1499,15 @@ __device__ __forceinline__ void flux_y_march_body(Dev<R> d, Geo g, Prm<R> p, con
     const int cd = c - 1;
     const bool dok = col && cd >= g.Jb - 1 && cd <= g.Je + 2;
     R Dn[NV];
+    bool exact = false;
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
-      const R Dv = din_val(w[0][q], w[1][q], w[2][q], w[3][q], mk[0], mk[1], mk[2], mk[3]);
-      Dn[q] = dok ? Dv : (R)0;
+    for (int q = 0; q < NV; q++) Dn[q] = din_val_hw(w[0][q], w[1][q], w[2][q], w[3][q], mk[0], mk[1], mk[2], mk[3], exact);
+    if (exact) {  // rare: a -0 slope t2 in a wet cell (one branch for all fields)
+#pragma unroll
+      for (int q = 0; q < NV; q++) Dn[q] = din_val_ref(w[0][q], w[1][q], w[2][q], w[3][q], mk[0], mk[1], mk[2], mk[3]);
     }
+#pragma unroll
+    for (int q = 0; q < NV; q++) Dn[q] = dok ? Dn[q] : (R)0;
     if (t >= 4) {
       // pair of cell m = c-2: right state of face m, left state of face m+1
       const int m = c - 2;
@@ -1550,7 +1555,7 @@ __device__ __forceinline__ void flux_y_march_body(Dev<R> d, Geo g, Prm<R> p, con
   }
 }
 template <class R, bool DISP, int RS>
-__global__ void __launch_bounds__(256, 5) k_flux_y_march(Dev<R> d, Geo g, Prm<R> p) {
+__global__ void __launch_bounds__(256, 4) k_flux_y_march(Dev<R> d, Geo g, Prm<R> p) {
   flux_y_march_body<R, DISP, RS>(d, g, p, grid_blk());
 }
 
