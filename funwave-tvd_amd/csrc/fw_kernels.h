@@ -1718,7 +1718,8 @@ __global__ void k_sources(Dev<R> d, Geo g, Prm<R> p, R AA) {
 // CAL_DISPERSION) instead of read from the planes the dispersion kernel
 // would otherwise write: the same operations on the same operands as there
 // (an interior cell is its own mirror cell; wall cells take 0 for the cross
-// derivatives).
+// derivatives).  XD is used only in production runs (FW_OPT_FULL_STATE 0),
+// so that variant carries no code for the full-state arrays.
 template <class R, bool FUSED, bool P2, bool XD = false>
 __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, R DT, R AA, const Blk &B) {
   const Tile tb = xcd_tile(B);
@@ -1777,7 +1778,11 @@ __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, 
     }
     R srcx, srcy, wmm;
     if (FUSED) {
-      source_terms<R, P2, XD>(d, g, p, AA, i, j, p.full_state != 0, p.full_state == 0, srcx, srcy, wmm, u1p, v1p);
+      // (XD is the production form: no full-state arrays, so none of their
+      // code or registers)
+      constexpr bool prod = XD;
+      source_terms<R, P2, XD>(d, g, p, AA, i, j, !prod && p.full_state != 0, prod || p.full_state == 0, srcx, srcy,
+                              wmm, u1p, v1p);
     } else {
       srcx = d.SourceX[o];
       srcy = d.SourceY[o];
@@ -1785,7 +1790,7 @@ __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, 
     }
     R pc = (R)0.5 * (p1 + p0);
     R qc = (R)0.5 * (q1 + q0);
-    if (p.full_state) {  // mean-flow diagnostics (etauv_solver.F:96-99, :247)
+    if (!XD && p.full_state) {  // mean-flow diagnostics (etauv_solver.F:96-99, :247)
       d.P_center[o] = pc;
       d.Q_center[o] = qc;
       d.U_davg[o] = ddiv(pc, rmax(hcur, mdf));
@@ -1797,7 +1802,7 @@ __device__ __forceinline__ void huv_body(Dev<R> d, Geo g, Prm<R> p, R al, R be, 
       R xmk = (R)(i - g.Ib) * p.DXg, ymk = (R)(j - g.Jb) * p.DYg;
       if (dabs(xmk - p.XC) < p.WW && dabs(ymk - p.YC) < half(p.YW)) r1 = r1 + wmm;
     }
-    if (p.full_state) d.Wsurf[o] = r1;
+    if (!XD && p.full_state) d.Wsurf[o] = r1;
     eta = al * eta0 + be * (eta + DT * r1);
     if (p.ETA_LIMITER) {
       if (eta < p.TROUGH) eta = p.TROUGH;
