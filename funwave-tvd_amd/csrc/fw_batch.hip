@@ -172,19 +172,19 @@ __global__ void __launch_bounds__(kDispTX * kDispTY) kb_disp_fused(FWB_ARGS, int
 template <class R, bool DISP>
 __global__ void __launch_bounds__(256) kb_flux_x(FWB_ARGS, int pm, int pa) {
   FWB_HEAD
-  flux_x_body<R, DISP>(mdev(M, pm, pa), M.g, M.p, tile_blk(e));
+  flux_x_body<R, DISP, true>(mdev(M, pm, pa), M.g, M.p, tile_blk(e));
 }
 
 template <class R, bool DISP, int RS>
-__global__ void __launch_bounds__(256, 5) kb_flux_y_march(FWB_ARGS, int pm, int pa) {
+__global__ void __launch_bounds__(256, 4) kb_flux_y_march(FWB_ARGS, int pm, int pa) {
   FWB_HEAD
-  flux_y_march_body<R, DISP, RS>(mdev(M, pm, pa), M.g, M.p, tile_blk(e));
+  flux_y_march_body<R, DISP, RS, true>(mdev(M, pm, pa), M.g, M.p, tile_blk(e));
 }
 
 template <class R, bool DISP, int K>
 __global__ void __launch_bounds__(64 * kFluxYThreadRows) kb_flux_y(FWB_ARGS, int pm, int pa) {
   FWB_HEAD
-  flux_y_body<R, DISP, K>(mdev(M, pm, pa), M.g, M.p, tile_blk(e));
+  flux_y_body<R, DISP, K, true>(mdev(M, pm, pa), M.g, M.p, tile_blk(e));
 }
 
 // SourceTerms + ESTIMATE_HUV of one stage; the wavemaker factors of the

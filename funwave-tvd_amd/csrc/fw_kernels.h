@@ -675,7 +675,8 @@ __device__ __forceinline__ void disp_terms_at(Dev<R> &d, const Geo &g, const Prm
          + (c12 * B1 * B1 - c16) * reta * reta * B2 * B2;
   const R u3 = -v * omega_1 - omega_0 * (ken1 * HUxyHVyy + ken2 * UxyVyy);
   const R v3 = u * omega_1 + omega_0 * (ken1 * HUxxHVxy + ken2 * UxxVxy);
-  if (p.full_state) {
+  // (the one-pass kernel, TT given, runs in production only)
+  if (std::is_same<TT, NoTT>::value && p.full_state) {
     d.U1pp[o] = u1pp; d.V1pp[o] = v1pp;
     d.U2[o] = u2; d.V2[o] = v2;
     d.U3[o] = u3; d.V3[o] = v3;
@@ -1133,14 +1134,17 @@ __device__ __forceinline__ void recon_x(R v0, R Dv, bool dok, bool pok, R &L, R 
 
 // Everything after the states: P, F, G, the wave speeds, HLL and the wall
 // and dry-cell conditions of one x face.
-template <class R>
+// PROD: a production-mode launch (FW_OPT_FULL_STATE 0), with no code for
+// the full-state arrays
+template <class R, bool PROD = false>
 __device__ __forceinline__ void xface_finish(const Dev<R> &d, const Geo &g, const Prm<R> &p, int i, int j, bool comp,
                                              R uL, R uR, R vL, R vR, R huL, R huR, R hvL, R hvR, R eL, R eR, R u4L,
                                              R u4R, R v4L, R v4R) {
   const size_t o = IX(i, j);
   const int M1 = g.M + 1;
   const R g1 = p.G1, g3 = p.G3;
-  if (comp && p.full_state) {
+  const bool fs = !PROD && p.full_state;
+  if (comp && fs) {
     d.UxL[o] = uL; d.UxR[o] = uR; d.VxL[o] = vL; d.VxR[o] = vR;
     d.HUxL[o] = huL; d.HUxR[o] = huR; d.HVxL[o] = hvL; d.HVxR[o] = hvR;
     d.EtaRxL[o] = eL; d.EtaRxR[o] = eR;
@@ -1148,7 +1152,7 @@ __device__ __forceinline__ void xface_finish(const Dev<R> &d, const Geo &g, cons
   }
   const R dpx = d.DepthX[o];
   R hL = eL + dpx, hR = eR + dpx;
-  if (p.full_state || !comp) { d.HxL[o] = hL; d.HxR[o] = hR; }
+  if (fs || !comp) { d.HxL[o] = hL; d.HxR[o] = hR; }
   const R m9u = (R)d.MASK9[IX(i <= g.M ? i : g.M, j)];
   R pl = huL + g1 * m9u * hL * u4L;
   R pr = huR + g1 * m9u * hR * u4R;
@@ -1156,7 +1160,7 @@ __device__ __forceinline__ void xface_finish(const Dev<R> &d, const Geo &g, cons
   R fxr = g3 * pr * (uR + g1 * m9u * u4R) + (R)0.5 * GRAV * (eR * eR * g3 + (R)2 * eR * dpx);
   R gxl = hL * (uL + g1 * m9u * u4L) * (vL + g1 * m9u * v4L) * g3;
   R gxr = hR * (uR + g1 * m9u * u4R) * (vR + g1 * m9u * v4R) * g3;
-  if (p.full_state || !comp) { d.PL[o] = pl; d.PR[o] = pr; d.FxL[o] = fxl; d.FxR[o] = fxr; d.GxL[o] = gxl; d.GxR[o] = gxr; }
+  if (fs || !comp) { d.PL[o] = pl; d.PR[o] = pr; d.FxL[o] = fxl; d.FxR[o] = fxr; d.GxL[o] = gxl; d.GxR[o] = gxr; }
   // wave speed of the nearest computed face (ghost faces copy it)
   int ci = i < 4 ? 4 : (i > M1 - 3 ? M1 - 3 : i);
   int cj = j < 4 ? 4 : (j > g.N - 3 ? g.N - 3 : j);
@@ -1170,7 +1174,7 @@ __device__ __forceinline__ void xface_finish(const Dev<R> &d, const Geo &g, cons
   } else {
     xface_speed(d, g, p, ci, cj, sl, sr);
   }
-  if (p.full_state) { d.SxL[o] = sl; d.SxR[o] = sr; }
+  if (fs) { d.SxL[o] = sl; d.SxR[o] = sr; }
   R P = hll(sl, sr, pl, pr, eL, eR);
   R Fx = hll(sl, sr, fxl, fxr, huL, huR);
   R Gx = hll(sl, sr, gxl, gxr, hvL, hvR);
@@ -1197,7 +1201,7 @@ __device__ __forceinline__ void xface_finish(const Dev<R> &d, const Geo &g, cons
 }
 
 // blockDim (64, 4): threadIdx.x is the lane, threadIdx.y picks the row.
-template <class R, bool DISP>
+template <class R, bool DISP, bool PROD = false>
 __device__ __forceinline__ void flux_x_body(Dev<R> d, Geo g, Prm<R> p, const Blk &B) {
   const int lane = threadIdx.x;
   const int c = B.x * kFluxXFaces + lane;  // cell / face index of this lane
@@ -1266,22 +1270,23 @@ __device__ __forceinline__ void flux_x_body(Dev<R> d, Geo g, Prm<R> p, const Blk
     eL = d.EtaRxL[o]; eR = d.EtaRxR[o];
   }
   if (!comp || !DISP) { u4L = d.U4xL[o]; u4R = d.U4xR[o]; v4L = d.V4xL[o]; v4R = d.V4xR[o]; }
-  xface_finish(d, g, p, c, j, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
+  xface_finish<R, PROD>(d, g, p, c, j, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
 }
-template <class R, bool DISP>
+template <class R, bool DISP, bool PROD = false>
 __global__ void __launch_bounds__(256) k_flux_x(Dev<R> d, Geo g, Prm<R> p) {
-  flux_x_body<R, DISP>(d, g, p, grid_blk());
+  flux_x_body<R, DISP, PROD>(d, g, p, grid_blk());
 }
 
 // y faces (CONSTRUCT_HO_Y fluxes.F:1642-1754, bc.F:147-210 and :221-293).
-template <class R>
+template <class R, bool PROD = false>
 __device__ __forceinline__ void yface_finish(const Dev<R> &d, const Geo &g, const Prm<R> &p, int i, int j, bool comp,
                                              R uL, R uR, R vL, R vR, R huL, R huR, R hvL, R hvR, R eL, R eR, R u4L,
                                              R u4R, R v4L, R v4R) {
   const size_t o = IX(i, j);
   const int N1 = g.N + 1;
   const R g1 = p.G1, g3 = p.G3;
-  if (comp && p.full_state) {
+  const bool fs = !PROD && p.full_state;
+  if (comp && fs) {
     d.UyL[o] = uL; d.UyR[o] = uR; d.VyL[o] = vL; d.VyR[o] = vR;
     d.HUyL[o] = huL; d.HUyR[o] = huR; d.HVyL[o] = hvL; d.HVyR[o] = hvR;
     d.EtaRyL[o] = eL; d.EtaRyR[o] = eR;
@@ -1289,7 +1294,7 @@ __device__ __forceinline__ void yface_finish(const Dev<R> &d, const Geo &g, cons
   }
   const R dpy = d.DepthY[o];
   R hL = eL + dpy, hR = eR + dpy;
-  if (p.full_state || !comp) { d.HyL[o] = hL; d.HyR[o] = hR; }
+  if (fs || !comp) { d.HyL[o] = hL; d.HyR[o] = hR; }
   const R m9v = (R)d.MASK9[IX(i, j <= g.N ? j : g.N)];
   R ql = hvL + g1 * m9v * hL * v4L;
   R qr = hvR + g1 * m9v * hR * v4R;
@@ -1297,7 +1302,7 @@ __device__ __forceinline__ void yface_finish(const Dev<R> &d, const Geo &g, cons
   R gyr = g3 * qr * (vR + g1 * m9v * v4R) + (R)0.5 * GRAV * (eR * eR * g3 + (R)2 * eR * dpy);
   R fyl = hL * (uL + g1 * m9v * u4L) * (vL + g1 * m9v * v4L) * g3;
   R fyr = hR * (uR + g1 * m9v * u4R) * (vR + g1 * m9v * v4R) * g3;
-  if (p.full_state || !comp) { d.QL[o] = ql; d.QR[o] = qr; d.GyL[o] = gyl; d.GyR[o] = gyr; d.FyL[o] = fyl; d.FyR[o] = fyr; }
+  if (fs || !comp) { d.QL[o] = ql; d.QR[o] = qr; d.GyL[o] = gyl; d.GyR[o] = gyr; d.FyL[o] = fyl; d.FyR[o] = fyr; }
   int ci = i < 4 ? 4 : (i > g.M - 3 ? g.M - 3 : i);
   int cj = j < 4 ? 4 : (j > N1 - 3 ? N1 - 3 : j);
   R sl, sr;
@@ -1310,7 +1315,7 @@ __device__ __forceinline__ void yface_finish(const Dev<R> &d, const Geo &g, cons
   } else {
     yface_speed(d, g, p, ci, cj, sl, sr);
   }
-  if (p.full_state) { d.SyL[o] = sl; d.SyR[o] = sr; }
+  if (fs) { d.SyL[o] = sl; d.SyR[o] = sr; }
   R Q = hll(sl, sr, ql, qr, eL, eR);
   R Fy = hll(sl, sr, fyl, fyr, huL, huR);
   R Gy = hll(sl, sr, gyl, gyr, hvL, hvR);
@@ -1352,7 +1357,7 @@ struct FluxYK {
   static constexpr int K = sizeof(R) == 4 ? 2 : 1;
 };
 
-template <class R, bool DISP, int K>
+template <class R, bool DISP, int K, bool PROD = false>
 __device__ __forceinline__ void flux_y_body(Dev<R> d, Geo g, Prm<R> p, const Blk &B) {
   constexpr int NV = DISP ? 7 : 5;
   constexpr int ROWS = FluxYTile<K>::rows, FACES = FluxYTile<K>::faces;
@@ -1439,12 +1444,12 @@ __device__ __forceinline__ void flux_y_body(Dev<R> d, Geo g, Prm<R> p, const Blk
     } else {
       u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
     }
-    yface_finish(d, g, p, i, j, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
+    yface_finish<R, PROD>(d, g, p, i, j, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
   }
 }
-template <class R, bool DISP, int K>
+template <class R, bool DISP, int K, bool PROD = false>
 __global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo g, Prm<R> p) {
-  flux_y_body<R, DISP, K>(d, g, p, grid_blk());
+  flux_y_body<R, DISP, K, PROD>(d, g, p, grid_blk());
 }
 
 // y faces by marching: each thread walks one column through a strip of RS
@@ -1459,7 +1464,7 @@ __global__ void __launch_bounds__(64 * kFluxYThreadRows) k_flux_y(Dev<R> d, Geo 
 // (four waves per SIMD: with the slopes' -0 fallback taken in one branch
 // for all fields, five waves spill and lose 20 %; four are 2 % faster than
 // the per-field branches at five)
-template <class R, bool DISP, int RS>
+template <class R, bool DISP, int RS, bool PROD = false>
 __device__ __forceinline__ void flux_y_march_body(Dev<R> d, Geo g, Prm<R> p, const Blk &B) {
   constexpr int NV = DISP ? 7 : 5;
   const int i = 1 + B.x * 64 + threadIdx.x;
@@ -1545,7 +1550,7 @@ __device__ __forceinline__ void flux_y_march_body(Dev<R> d, Geo g, Prm<R> p, con
         } else {
           u4L = d.U4yL[o]; u4R = d.U4yR[o]; v4L = d.V4yL[o]; v4R = d.V4yR[o];
         }
-        yface_finish(d, g, p, i, m, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
+        yface_finish<R, PROD>(d, g, p, i, m, comp, uL, uR, vL, vR, huL, huR, hvL, hvR, eL, eR, u4L, u4R, v4L, v4R);
       }
 #pragma unroll
       for (int q = 0; q < NV; q++) Lp[q] = Ln[q];
@@ -1554,9 +1559,9 @@ __device__ __forceinline__ void flux_y_march_body(Dev<R> d, Geo g, Prm<R> p, con
     for (int q = 0; q < NV; q++) Dp[q] = Dn[q];
   }
 }
-template <class R, bool DISP, int RS>
+template <class R, bool DISP, int RS, bool PROD = false>
 __global__ void __launch_bounds__(256, 4) k_flux_y_march(Dev<R> d, Geo g, Prm<R> p) {
-  flux_y_march_body<R, DISP, RS>(d, g, p, grid_blk());
+  flux_y_march_body<R, DISP, RS, PROD>(d, g, p, grid_blk());
 }
 
 // --------------------------------------------------------------------------

@@ -419,35 +419,41 @@ int do_dispersion(fw_ctx *c, bool in_step = false) {
   return FW_OK;
 }
 
-template <class R>
-int do_fluxes(fw_ctx *c) {
+// production launches (FW_OPT_FULL_STATE 0) take the flux kernels without
+// the full-state arrays' code
+template <class R, bool PROD>
+int do_fluxes_as(fw_ctx *c) {
   Dev<R> &d = c->dev<R>();
   const Geo &g = c->g;
   const dim3 gx((g.M + 1 + kFluxXFaces - 1) / kFluxXFaces, (g.N + kB2.y - 1) / kB2.y);
   const bool disp = c->cfg.DISPERSION;
   if (disp)
-    launch(c, "flux_x", k_flux_x<R, true>, gx, kB2, d, g, c->prm<R>());
+    launch(c, "flux_x", k_flux_x<R, true, PROD>, gx, kB2, d, g, c->prm<R>());
   else
-    launch(c, "flux_x", k_flux_x<R, false>, gx, kB2, d, g, c->prm<R>());
+    launch(c, "flux_x", k_flux_x<R, false, PROD>, gx, kB2, d, g, c->prm<R>());
   if constexpr (sizeof(R) == 4) {
-    // FP32: columns marched in strips of 4 faces (96 VGPRs, independent waves)
+    // FP32: columns marched in strips of 4 faces (independent waves)
     constexpr int RS = 4;
     const int nt = (g.N + 1 + RS - 1) / RS;
     const dim3 by(64, 4), gy((g.M + 63) / 64, (nt + 3) / 4);
     if (disp)
-      launch(c, "flux_y", k_flux_y_march<R, true, RS>, gy, by, d, g, c->prm<R>());
+      launch(c, "flux_y", k_flux_y_march<R, true, RS, PROD>, gy, by, d, g, c->prm<R>());
     else
-      launch(c, "flux_y", k_flux_y_march<R, false, RS>, gy, by, d, g, c->prm<R>());
+      launch(c, "flux_y", k_flux_y_march<R, false, RS, PROD>, gy, by, d, g, c->prm<R>());
   } else {
     // FP64: the LDS tile (the marching window would need 216 VGPRs)
     constexpr int KY = FluxYK<R>::K, FY = FluxYTile<KY>::faces;
     const dim3 gy((g.M + 63) / 64, (g.N + 1 + FY - 1) / FY), by(64, kFluxYThreadRows);
     if (disp)
-      launch(c, "flux_y", k_flux_y<R, true, KY>, gy, by, d, g, c->prm<R>());
+      launch(c, "flux_y", k_flux_y<R, true, KY, PROD>, gy, by, d, g, c->prm<R>());
     else
-      launch(c, "flux_y", k_flux_y<R, false, KY>, gy, by, d, g, c->prm<R>());
+      launch(c, "flux_y", k_flux_y<R, false, KY, PROD>, gy, by, d, g, c->prm<R>());
   }
   return FW_OK;
+}
+template <class R>
+int do_fluxes(fw_ctx *c) {
+  return c->prm<R>().full_state ? do_fluxes_as<R, false>(c) : do_fluxes_as<R, true>(c);
 }
 
 template <class R>
