@@ -575,7 +575,9 @@ int do_solve(fw_ctx *c) {
       int nxblk = (ny + bs - 1) / bs, nyblk = (nx + bs - 1) / bs;
       launch(c, "trid", k_trid<R>, dim3(nxblk + nyblk), dim3(bs), d, g, c->prm<R>(), nxblk);
     } else {
-      constexpr int L = sizeof(R) == 4 ? 16 : 8, CH = 64;
+      // 16 lines per workgroup (2 x 2048 lines: one workgroup per CU);
+      // FP64 rows take twice the LDS, so its chunks are 48 rows (125 KB)
+      constexpr int L = 16, CH = sizeof(R) == 4 ? 64 : 48;
       int nxblk = (ny + L - 1) / L, nyblk = (nx + L - 1) / L;
       launch(c, "trid", k_trid_lds<R, L, CH>, dim3(nxblk + nyblk), dim3(256), d, g, nxblk);
     }
