@@ -204,6 +204,9 @@ __global__ void __launch_bounds__(WideLB<R>::wg256) kb_huv(FWB_ARGS, int pm, int
   huv_body<R, true, P2, XD>(d, M.g, M.p, al, be, M.clk[0], AA, tile_blk(e));
 }
 
+// lines per workgroup of the line solves, as in fw_step (FP64 in 48-row
+// chunks, so that its LDS tile fits)
+constexpr int kTridLines = 16;
 template <class R, int L, int CH>
 __global__ void __launch_bounds__(256) kb_trid_lds(FWB_ARGS, int pm, int pa) {
   FWB_HEAD
@@ -531,7 +534,7 @@ int build_queues(fw_batch *b) {
     auto ny = [&](int m) { const Geo &g = mem(m).g; return (g.Ie - g.Ib + 1 + 63) / 64; };
     if ((rc = make_line(b, "trid", dim3(64), [&](int m) { return nx(m) + ny(m); }, nx))) return rc;
   } else {
-    constexpr int L = sizeof(R) == 4 ? 16 : 8;
+    constexpr int L = kTridLines;
     auto nx = [&](int m) { const Geo &g = mem(m).g; return (g.Je - g.Jb + 1 + L - 1) / L; };
     auto ny = [&](int m) { const Geo &g = mem(m).g; return (g.Ie - g.Ib + 1 + L - 1) / L; };
     if ((rc = make_line(b, "trid", dim3(256), [&](int m) { return nx(m) + ny(m); }, nx))) return rc;
@@ -608,7 +611,7 @@ int rec_step(fw_batch *b, int p0) {
     BCHK(b, hipStreamWaitEvent(sd, b->ev_fork, 0));
     if (b->disp) {
       if (b->periodic) bl<R>(b, s, "trid", kb_trid<R>, par, par);
-      else bl<R>(b, s, "trid", kb_trid_lds<R, sizeof(R) == 4 ? 16 : 8, 64>, par, par);
+      else bl<R>(b, s, "trid", kb_trid_lds<R, kTridLines, sizeof(R) == 4 ? 64 : 48>, par, par);
     }
     bl<R>(b, s, "cell", kb_uv_final<R>, par, par);
     if (!b->disp) bl<R>(b, s, "copy", kb_ubar_from_hu<R>);
