@@ -2577,6 +2577,10 @@ __global__ void __launch_bounds__(1024) k_brk2(Dev<R> d, Geo g, Prm<R> p, R DT) 
 // --------------------------------------------------------------------------
 // EXCHANGE tail (bc.F:491-494): U,V,HU,HV *= MASK on all cells, optionally
 // followed by SPONGE_DAMPING (sponge.F:52-66).
+__device__ __forceinline__ bool bits_equal(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
+__device__ __forceinline__ bool bits_equal(double a, double b) {
+  return __double_as_longlong(a) == __double_as_longlong(b);
+}
 template <class R>
 __device__ __forceinline__ void mask_mul_body(Dev<R> d, Geo g, Prm<R> p, int sponge, int mul, const Blk &B) {
   int i = 1 + B.x * blockDim.x + threadIdx.x;  // pointwise: launch order
@@ -2589,21 +2593,26 @@ __device__ __forceinline__ void mask_mul_body(Dev<R> d, Geo g, Prm<R> p, int spo
   // x*1 and x/1 leave every bit of x unchanged: cells with MASK = 1 and
   // SPONGE = 1 need no work
   if (mk == 1 && sp == (R)1) return;
-  R u = d.U[o], v = d.V[o];
+  // a value whose bits the multiply leaves as they are is not stored again
+  // (dry cells already hold the products by 0 of the last multiply)
+  const R u0 = d.U[o], v0 = d.V[o];
+  R u = u0, v = v0;
   if (mul) {
     R m = (R)mk;
     u = u * m;
     v = v * m;
-    d.HU[o] = d.HU[o] * m;
-    d.HV[o] = d.HV[o] * m;
+    const R hu = d.HU[o], hv = d.HV[o];
+    const R hum = hu * m, hvm = hv * m;
+    if (!bits_equal(hum, hu)) d.HU[o] = hum;
+    if (!bits_equal(hvm, hv)) d.HV[o] = hvm;
   }
   if (sponge) {
     if ((R)mk > (R)0) d.Eta[o] = ddiv(d.Eta[o], sp);
     u = ddiv(u, sp);
     v = ddiv(v, sp);
   }
-  d.U[o] = u;
-  d.V[o] = v;
+  if (!bits_equal(u, u0)) d.U[o] = u;
+  if (!bits_equal(v, v0)) d.V[o] = v;
 }
 template <class R>
 __global__ void k_mask_mul(Dev<R> d, Geo g, Prm<R> p, int sponge, int mul) {
